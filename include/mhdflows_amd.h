@@ -1,0 +1,189 @@
+/*
+ * MI355X-native MHDFlows hot path: C ABI of libmhdflows_amd.so.
+ *
+ * The library implements the 3D periodic incompressible HD/MHD right-hand side
+ * and its RK4 / LSRK54 time step of MHDFlows.jl v0.2.2 on one AMD Instinct
+ * MI355X (gfx950).  Every entry point takes plain pointers and sizes; no C++ or
+ * torch types cross this boundary.  All functions return 0 on success or a
+ * negative MHD_ERR_* code; mhd_last_error() gives the message of the calling
+ * thread's last failure.
+ *
+ * Array layout at the boundary is byte-identical to the reference's Julia
+ * arrays: the state `sol` is Complex{T}[kr, l, m, f] column-major, i.e. C order
+ * [f][z][y][kx] with nkr = nx/2+1 and complex values interleaved (re, im); real
+ * fields are T[x, y, z] column-major, i.e. C order [z][y][x].  Field order
+ * f = ux, uy, uz, bx, by, bz (src/Structure/datastructure.jl:90).
+ *
+ * Ownership: a handle owns all of its device memory.  Host pointers passed in are
+ * copied before the call returns and never retained.  Device pointers passed in
+ * (mhd_calcN and friends) must hold the documented number of fields.
+ *
+ * Threading: one handle is used by one host thread at a time; distinct handles are
+ * independent.  Work is ordered on the handle's HIP stream.  Calls that return
+ * host data synchronize that stream.
+ */
+#ifndef MHDFLOWS_AMD_H
+#define MHDFLOWS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD_ABI_VERSION 1
+
+enum mhd_status {
+  MHD_OK = 0,
+  MHD_ERR_INVALID = -1,      /* invalid argument (e.g. odd grid size, bad field index) */
+  MHD_ERR_UNSUPPORTED = -2,  /* grid length without a compiled transform */
+  MHD_ERR_BUDGET = -3,       /* device memory needed exceeds the configured HBM budget */
+  MHD_ERR_HIP = -4,          /* HIP runtime error (out of memory, launch failure, no device) */
+  MHD_ERR_NAN = -5,          /* NaN detected in the fields */
+};
+
+enum mhd_precision { MHD_F32 = 0, MHD_F64 = 1 };
+enum mhd_physics { MHD_PHYS_HD = 0, MHD_PHYS_MHD = 1 };
+enum mhd_stepper { MHD_RK4 = 0, MHD_LSRK54 = 1 };
+
+typedef struct mhd_handle mhd_handle;
+
+/* Problem(dev; nx, ny, nz, Lx, Ly, Lz, ν, nν, η, nη, B_field, stepper, T, dt)
+ * -- src/pgen.jl:64-127.  As in the reference, nη is not forwarded to the
+ * parameters (src/pgen.jl:114-116), so the resistive hyper term stays off. */
+typedef struct mhd_config {
+  int32_t nx, ny, nz;        /* even grid sizes */
+  double Lx, Ly, Lz;         /* box lengths (2π by default in the reference) */
+  double nu, eta;            /* viscosity / resistivity, kept in Float64 (MHDParams.jl:45-47) */
+  int32_t n_nu, n_eta;       /* hyper-viscosity order (active when > 1) */
+  int32_t precision;         /* mhd_precision: T = Float32 or Float64 */
+  int32_t physics;           /* mhd_physics: B_field = false / true */
+  int32_t stepper;           /* mhd_stepper */
+  int32_t device;            /* HIP device ordinal */
+  void* stream;              /* hipStream_t to order work on, or NULL for a library-owned stream */
+  uint64_t hbm_budget;       /* bytes of device memory the handle may use; 0 = no limit */
+  double dt;                 /* initial clock.dt */
+} mhd_config;
+
+/* Diagnostics in Float64 (ProbDiagnostic, src/utils/UserInterface.jl:65-86, without
+ * the 3-significant-digit display rounding; helicities as src/utils/MHDAnalysis.jl:94-174). */
+typedef struct mhd_diag {
+  double E_kin;      /* Σ|u|² dV (no 1/2, as the reference) */
+  double E_mag;      /* Σ|b|² dV */
+  double H_kin;      /* Σ u·(∇×u) dV */
+  double H_mag;      /* Σ A·b dV, Coulomb gauge */
+  double H_cross;    /* Σ u·b dV */
+  double max_div_b;  /* max |∇·b| (Div, src/utils/VectorCalculus.jl:72-94) */
+  double max_div_u;  /* max |∇·u| */
+  int32_t has_nan;   /* 1 if E_kin or E_mag is NaN */
+  int32_t pad;
+} mhd_diag;
+
+/* Per-kernel timing collected with HIP events on the handle's stream. */
+typedef struct mhd_ktime {
+  char name[32];
+  int64_t launches;
+  double total_ms;
+  double bytes;      /* algorithmic bytes moved by all timed launches of this kernel */
+} mhd_ktime;
+
+/* ---- library ---------------------------------------------------------------- */
+int32_t mhd_abi_version(void);
+const char* mhd_last_error(void);
+/* 1 if a transform of length n is compiled in (n even, 8..1024, factors 2 and 3). */
+int32_t mhd_supported_length(int32_t n);
+
+/* ---- problem lifetime: Problem(dev; ...) src/pgen.jl:64-127 ------------------ */
+/* Device bytes a handle with this configuration allocates. */
+int32_t mhd_query_bytes(const mhd_config* cfg, uint64_t* bytes);
+int32_t mhd_create(const mhd_config* cfg, mhd_handle** out);
+int32_t mhd_destroy(mhd_handle* h);
+
+/* ---- state boundary ----------------------------------------------------------- */
+/* SetUpProblemIC!(prob; ux..bz) src/utils/IC.jl:41-109: copy real host fields
+ * (nf = 3 HD / 6 MHD entries, each nx*ny*nz values of T; NULL entries are skipped)
+ * into vars and R2C them into sol. */
+int32_t mhd_set_ic_real(mhd_handle* h, const void* const* fields);
+/* Copy prob.sol (nf*nz*ny*nkr complex T) from / to host memory. */
+int32_t mhd_get_sol(mhd_handle* h, void* host_out);
+int32_t mhd_set_sol(mhd_handle* h, const void* host_in);
+/* Copy prob.vars real field f (0..nf-1) to host memory. */
+int32_t mhd_get_vars(mhd_handle* h, int32_t field, void* host_out);
+/* Device pointers of the handle's sol (nf fields) and vars (nf real fields). */
+int32_t mhd_device_ptrs(mhd_handle* h, void** sol, void** vars);
+/* Clock (src/Problems.jl:53-60): t and dt as stored in T, step counter. */
+int32_t mhd_get_clock(mhd_handle* h, double* t, double* dt, int64_t* step);
+int32_t mhd_set_clock(mhd_handle* h, double t, double dt, int64_t step);
+
+/* ---- operator boundary (device pointers, reference layout) --------------------- */
+/* MHDcalcN!/HDcalcN!(N, sol, t, ...) src/pgen.jl:153-181: dealias sol in place,
+ * refresh vars, write the right-hand side of all nf fields into N (forcing
+ * added for MHD after the advection; for HD it is added first and then
+ * discarded, as in the reference). */
+int32_t mhd_calcN(mhd_handle* h, void* N_dev, void* sol_dev, double t);
+/* MHDcalcN_advection! / HDcalcN_advection! src/Solver/MHDSolver.jl:330-351,
+ * src/Solver/HDSolver.jl:95-108: no dealiasing, no forcing. */
+int32_t mhd_advection(mhd_handle* h, void* N_dev, void* sol_dev, double t);
+/* UᵢUpdate!(N, ...; direction) src/Solver/MHDSolver.jl:27-103 (HD: HDSolver.jl:25-93)
+ * and BᵢUpdate! MHDSolver.jl:106-177: read the handle's vars only and write
+ * component dir (0=x, 1=y, 2=z) of N (field dir, resp. 3+dir). */
+int32_t mhd_u_update(mhd_handle* h, void* N_dev, int32_t dir);
+int32_t mhd_b_update(mhd_handle* h, void* N_dev, int32_t dir);
+
+/* ---- time stepping ------------------------------------------------------------ */
+/* stepforward! src/timestepper/timestepper.jl:4-6 with the configured stepper,
+ * nsteps times with the current clock.dt (no host synchronisation). */
+int32_t mhd_step(mhd_handle* h, int32_t nsteps);
+/* getCFL! src/integrator.jl:158-198: set clock.dt from the current vars. */
+int32_t mhd_cfl_dt(mhd_handle* h, double cfl_coef, double* dt_out);
+/* TimeIntegrator!(prob, t0, N0; usr_dt, CFL_Coef) hot loop src/integrator.jl:31-156:
+ * clock.step = 0, then while N0 >= step && t0 >= t: [getCFL! unless usr_dt != 0],
+ * stepforward!.  dt is computed on the device; with t0 = +Inf no host
+ * synchronisation happens inside the loop.  Returns the number of steps taken in
+ * *steps_out (may be NULL). */
+int32_t mhd_integrate(mhd_handle* h, double t0, int64_t N0, double usr_dt, double cfl_coef,
+                      int64_t* steps_out);
+/* Capture one step into a HIP graph and replay it for subsequent mhd_step calls
+ * (enable = 1) or launch kernels one by one (enable = 0). */
+int32_t mhd_use_graph(mhd_handle* h, int32_t enable);
+
+/* ---- diagnostics and forcing -------------------------------------------------- */
+/* FP64 reductions over the current vars (the reference's diagnostics read vars). */
+int32_t mhd_diagnostics(mhd_handle* h, mhd_diag* out);
+/* Refresh vars from sol (C2R of the dealiased state) without stepping. */
+int32_t mhd_refresh_vars(mhd_handle* h);
+/* calcF! hook (src/pgen.jl:231-234): seeded time-independent solenoidal forcing on
+ * the shell round(|k|) == kf with amplitude amp (seed 0 and amp 0 turn it off). */
+int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp);
+
+/* ---- measurement -------------------------------------------------------------- */
+/* Time nsteps steps with HIP events on the handle's stream (whole-step wall time). */
+int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out);
+/* Run nsteps steps with an event pair around every kernel launch and report per
+ * kernel class launches, time and algorithmic bytes (at most max entries). */
+int32_t mhd_profile_steps(mhd_handle* h, int32_t nsteps, mhd_ktime* out, int32_t max, int32_t* n_out);
+/* Algorithmic bytes of one step (the 3-pass model: MHD RK4 = 438 V_c + 120 V_r). */
+double mhd_step_bytes_model(const mhd_config* cfg);
+
+/* ---- batched ensembles: many independent runs stepped together ----------------- */
+typedef struct mhd_batch mhd_batch;
+/* Runs are grouped by (grid size, precision); each group steps with one launch per
+ * kernel class covering all its runs, and the groups' launches are captured into
+ * one HIP graph per ensemble step. */
+int32_t mhd_batch_create(const mhd_config* cfgs, int32_t nruns, mhd_batch** out);
+int32_t mhd_batch_destroy(mhd_batch* b);
+int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields);
+int32_t mhd_batch_set_sol(mhd_batch* b, int32_t run, const void* host_in);
+int32_t mhd_batch_get_sol(mhd_batch* b, int32_t run, void* host_out);
+int32_t mhd_batch_get_clock(mhd_batch* b, int32_t run, double* t, double* dt, int64_t* step);
+/* Set per-run CFL control (coef > 0: dt from getCFL! before every step). */
+int32_t mhd_batch_set_cfl(mhd_batch* b, int32_t run, double cfl_coef);
+/* Advance every run by nsteps steps. */
+int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps);
+int32_t mhd_batch_use_graph(mhd_batch* b, int32_t enable);
+int32_t mhd_batch_time_steps(mhd_batch* b, int32_t nsteps, double* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHDFLOWS_AMD_H */

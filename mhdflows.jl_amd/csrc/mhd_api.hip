@@ -1,0 +1,363 @@
+// extern "C" entry points of libmhdflows_amd.so (declared in include/mhdflows_amd.h).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+
+#include "../../include/mhdflows_amd.h"
+#include "mhd_engine.hpp"
+
+using namespace mhd;
+
+struct mhd_handle {
+  mhd_config cfg;
+  EngineBase* e;
+};
+
+struct mhd_batch {
+  struct Group {
+    EngineBase* e;
+    std::vector<int> runs;  // global run ids in this group
+  };
+  std::vector<Group> groups;
+  std::vector<std::pair<int, int>> where;  // global run -> (group, local run)
+  std::vector<mhd_config> cfgs;
+};
+
+namespace {
+
+int validate(const mhd_config* c) {
+  if (!c) return fail(E_INVALID, "config is NULL");
+  const int n[3] = {c->nx, c->ny, c->nz};
+  for (int v : n) {
+    if (v <= 0 || v % 2) return fail(E_INVALID, "grid sizes must be positive and even (Int(nx/2+1), utils.jl:20)");
+    if (!length_supported(v))
+      return fail(E_UNSUPPORTED, "grid length " + std::to_string(v) + " has no compiled transform");
+  }
+  if (!(c->Lx > 0 && c->Ly > 0 && c->Lz > 0)) return fail(E_INVALID, "box lengths must be positive");
+  if (c->precision != MHD_F32 && c->precision != MHD_F64) return fail(E_INVALID, "precision must be F32 or F64");
+  if (c->physics != MHD_PHYS_HD && c->physics != MHD_PHYS_MHD) return fail(E_INVALID, "physics must be HD or MHD");
+  if (c->stepper != MHD_RK4 && c->stepper != MHD_LSRK54) return fail(E_INVALID, "stepper must be RK4 or LSRK54");
+  if (c->nu < 0 || c->eta < 0) return fail(E_INVALID, "viscosity and resistivity must be >= 0");
+  return 0;
+}
+
+Geom geom_of(const mhd_config* c) { return make_geom(c->nx, c->ny, c->nz, c->Lx, c->Ly, c->Lz); }
+
+RunParams params_of(const mhd_config* c) {
+  RunParams p;
+  p.nu = c->nu;
+  p.eta = c->physics == MHD_PHYS_MHD ? c->eta : 0.0;
+  p.nnu = c->n_nu;
+  p.neta = 0;  // nη is not forwarded to SetParams (src/pgen.jl:114-116)
+  p.dt = c->dt;
+  return p;
+}
+
+int check_handle(mhd_handle* h) {
+  if (!h || !h->e) return fail(E_INVALID, "handle is NULL");
+  hipSetDevice(h->cfg.device);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mhd_abi_version(void) { return MHD_ABI_VERSION; }
+const char* mhd_last_error(void) { return g_err.c_str(); }
+int32_t mhd_supported_length(int32_t n) { return length_supported(n) ? 1 : 0; }
+
+int32_t mhd_query_bytes(const mhd_config* cfg, uint64_t* bytes) {
+  MHD_TRY(validate(cfg));
+  if (!bytes) return fail(E_INVALID, "bytes is NULL");
+  *bytes = (uint64_t)run_bytes(geom_of(cfg), cfg->physics, cfg->precision);
+  return 0;
+}
+
+int32_t mhd_create(const mhd_config* cfg, mhd_handle** out) {
+  MHD_TRY(validate(cfg));
+  if (!out) return fail(E_INVALID, "out is NULL");
+  *out = nullptr;
+  const Geom g = geom_of(cfg);
+  const uint64_t need = (uint64_t)run_bytes(g, cfg->physics, cfg->precision);
+  if (cfg->hbm_budget && need > cfg->hbm_budget)
+    return fail(E_BUDGET, "problem needs " + std::to_string(need) + " bytes, budget is " +
+                              std::to_string(cfg->hbm_budget));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(E_HIP, "no HIP device available");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(E_INVALID, "device ordinal out of range");
+  const double L[3] = {cfg->Lx, cfg->Ly, cfg->Lz};
+  int err = 0;
+  EngineBase* e = make_engine(cfg->precision, g, L, cfg->physics, cfg->stepper, {params_of(cfg)}, cfg->device,
+                              (hipStream_t)cfg->stream, &err);
+  if (!e) return err ? err : fail(E_HIP, "engine creation failed");
+  *out = new mhd_handle{*cfg, e};
+  return 0;
+}
+
+int32_t mhd_destroy(mhd_handle* h) {
+  if (!h) return 0;
+  if (h->e) {
+    hipSetDevice(h->cfg.device);
+    delete h->e;
+  }
+  delete h;
+  return 0;
+}
+
+int32_t mhd_set_ic_real(mhd_handle* h, const void* const* fields) {
+  MHD_TRY(check_handle(h));
+  if (!fields) return fail(E_INVALID, "fields is NULL");
+  return h->e->set_ic_real(0, fields);
+}
+int32_t mhd_get_sol(mhd_handle* h, void* host_out) {
+  MHD_TRY(check_handle(h));
+  if (!host_out) return fail(E_INVALID, "output pointer is NULL");
+  return h->e->get_sol(0, host_out);
+}
+int32_t mhd_set_sol(mhd_handle* h, const void* host_in) {
+  MHD_TRY(check_handle(h));
+  if (!host_in) return fail(E_INVALID, "input pointer is NULL");
+  return h->e->set_sol(0, host_in);
+}
+int32_t mhd_get_vars(mhd_handle* h, int32_t field, void* host_out) {
+  MHD_TRY(check_handle(h));
+  if (!host_out) return fail(E_INVALID, "output pointer is NULL");
+  return h->e->get_vars(0, field, host_out);
+}
+int32_t mhd_device_ptrs(mhd_handle* h, void** sol, void** vars) {
+  MHD_TRY(check_handle(h));
+  return h->e->device_ptrs(0, sol, vars);
+}
+int32_t mhd_get_clock(mhd_handle* h, double* t, double* dt, int64_t* step) {
+  MHD_TRY(check_handle(h));
+  return h->e->get_clock(0, t, dt, step);
+}
+int32_t mhd_set_clock(mhd_handle* h, double t, double dt, int64_t step) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_clock(0, t, dt, step);
+}
+
+int32_t mhd_calcN(mhd_handle* h, void* N_dev, void* sol_dev, double /*t*/) {
+  MHD_TRY(check_handle(h));
+  if (!N_dev || !sol_dev) return fail(E_INVALID, "N and sol must be device pointers");
+  return h->e->calcN(N_dev, sol_dev, 0);
+}
+int32_t mhd_advection(mhd_handle* h, void* N_dev, void* sol_dev, double /*t*/) {
+  MHD_TRY(check_handle(h));
+  if (!N_dev || !sol_dev) return fail(E_INVALID, "N and sol must be device pointers");
+  return h->e->calcN(N_dev, sol_dev, 1);
+}
+int32_t mhd_u_update(mhd_handle* h, void* N_dev, int32_t dir) {
+  MHD_TRY(check_handle(h));
+  if (!N_dev) return fail(E_INVALID, "N must be a device pointer");
+  if (dir < 0 || dir > 2) return fail(E_INVALID, "Warning : Unknown direction is declerad");  // MHDSolver.jl:52
+  return h->e->component_update(N_dev, dir);
+}
+int32_t mhd_b_update(mhd_handle* h, void* N_dev, int32_t dir) {
+  MHD_TRY(check_handle(h));
+  if (!N_dev) return fail(E_INVALID, "N must be a device pointer");
+  if (h->cfg.physics != MHD_PHYS_MHD) return fail(E_INVALID, "BᵢUpdate! needs B_field = true");
+  if (dir < 0 || dir > 2) return fail(E_INVALID, "Warning : Unknown direction is declerad");
+  return h->e->component_update(N_dev, 3 + dir);
+}
+
+int32_t mhd_step(mhd_handle* h, int32_t nsteps) {
+  MHD_TRY(check_handle(h));
+  return h->e->step(nsteps);
+}
+int32_t mhd_cfl_dt(mhd_handle* h, double cfl_coef, double* dt_out) {
+  MHD_TRY(check_handle(h));
+  return h->e->cfl_now(cfl_coef, dt_out);
+}
+int32_t mhd_integrate(mhd_handle* h, double t0, int64_t N0, double usr_dt, double cfl_coef, int64_t* steps_out) {
+  MHD_TRY(check_handle(h));
+  return h->e->integrate(t0, N0, usr_dt, cfl_coef, steps_out);
+}
+int32_t mhd_use_graph(mhd_handle* h, int32_t enable) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_graph(enable);
+}
+
+int32_t mhd_diagnostics(mhd_handle* h, mhd_diag* out) {
+  MHD_TRY(check_handle(h));
+  if (!out) return fail(E_INVALID, "out is NULL");
+  double v[8];
+  MHD_TRY(h->e->diagnostics(v));
+  out->E_kin = v[0];
+  out->E_mag = v[1];
+  out->H_kin = v[2];
+  out->H_mag = v[3];
+  out->H_cross = v[4];
+  out->max_div_b = v[5];
+  out->max_div_u = v[6];
+  out->has_nan = v[7] != 0.0;
+  out->pad = 0;
+  return 0;
+}
+int32_t mhd_refresh_vars(mhd_handle* h) {
+  MHD_TRY(check_handle(h));
+  return h->e->refresh_vars();
+}
+int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp) {
+  MHD_TRY(check_handle(h));
+  if (!(kf >= 0)) return fail(E_INVALID, "kf must be >= 0");
+  return h->e->set_forcing(0, seed, kf, amp);
+}
+
+int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out) {
+  MHD_TRY(check_handle(h));
+  if (!ms_out) return fail(E_INVALID, "ms_out is NULL");
+  return h->e->time_steps(nsteps, ms_out);
+}
+int32_t mhd_profile_steps(mhd_handle* h, int32_t nsteps, mhd_ktime* out, int32_t max, int32_t* n_out) {
+  MHD_TRY(check_handle(h));
+  std::vector<KTime> k;
+  MHD_TRY(h->e->profile_steps(nsteps, &k));
+  const int n = std::min<int>((int)k.size(), max);
+  for (int i = 0; i < n; ++i) {
+    std::memset(out[i].name, 0, sizeof(out[i].name));
+    std::strncpy(out[i].name, k[i].name.c_str(), sizeof(out[i].name) - 1);
+    out[i].launches = k[i].launches;
+    out[i].total_ms = k[i].ms;
+    out[i].bytes = k[i].bytes;
+  }
+  if (n_out) *n_out = n;
+  return 0;
+}
+
+double mhd_step_bytes_model(const mhd_config* cfg) {
+  if (!cfg) return 0.0;
+  const double s = cfg->precision == MHD_F32 ? 4.0 : 8.0;
+  const double n3 = (double)cfg->nx * cfg->ny * cfg->nz;
+  const double Vr = s * n3;
+  const double Vc = 2.0 * s * (cfg->nx / 2 + 1) * (double)cfg->ny * cfg->nz;
+  if (cfg->physics == MHD_PHYS_MHD)
+    return cfg->stepper == MHD_RK4 ? 438 * Vc + 120 * Vr : 534 * Vc + 150 * Vr;
+  return cfg->stepper == MHD_RK4 ? 255 * Vc + 72 * Vr : 0.0;
+}
+
+// ---- batched ensembles -----------------------------------------------------------
+int32_t mhd_batch_create(const mhd_config* cfgs, int32_t nruns, mhd_batch** out) {
+  if (!cfgs || nruns <= 0 || !out) return fail(E_INVALID, "need at least one config");
+  *out = nullptr;
+  uint64_t need = 0, budget = 0;
+  for (int i = 0; i < nruns; ++i) {
+    MHD_TRY(validate(&cfgs[i]));
+    need += (uint64_t)run_bytes(geom_of(&cfgs[i]), cfgs[i].physics, cfgs[i].precision);
+    if (cfgs[i].hbm_budget) budget = cfgs[i].hbm_budget;
+  }
+  if (budget && need > budget) return fail(E_BUDGET, "ensemble exceeds the HBM budget");
+  // group by everything that fixes the kernel instances and launch shapes
+  using Key = std::tuple<int, int, int, double, double, double, int, int, int, int>;
+  std::map<Key, int> gid;
+  auto* b = new mhd_batch();
+  b->cfgs.assign(cfgs, cfgs + nruns);
+  std::vector<std::vector<int>> members;
+  for (int i = 0; i < nruns; ++i) {
+    const mhd_config& c = cfgs[i];
+    Key k{c.nx, c.ny, c.nz, c.Lx, c.Ly, c.Lz, c.precision, c.physics, c.stepper, c.device};
+    if (!gid.count(k)) {
+      gid[k] = (int)members.size();
+      members.emplace_back();
+    }
+    members[gid[k]].push_back(i);
+  }
+  b->where.resize(nruns);
+  for (size_t gi = 0; gi < members.size(); ++gi) {
+    const mhd_config& c0 = cfgs[members[gi][0]];
+    std::vector<RunParams> rp;
+    for (size_t j = 0; j < members[gi].size(); ++j) {
+      rp.push_back(params_of(&cfgs[members[gi][j]]));
+      b->where[members[gi][j]] = {(int)gi, (int)j};
+    }
+    const double L[3] = {c0.Lx, c0.Ly, c0.Lz};
+    int err = 0;
+    EngineBase* e = make_engine(c0.precision, geom_of(&c0), L, c0.physics, c0.stepper, rp, c0.device, nullptr, &err);
+    if (!e) {
+      mhd_batch_destroy(b);
+      return err ? err : fail(E_HIP, "engine creation failed");
+    }
+    b->groups.push_back({e, members[gi]});
+  }
+  *out = b;
+  return 0;
+}
+
+int32_t mhd_batch_destroy(mhd_batch* b) {
+  if (!b) return 0;
+  for (auto& g : b->groups) delete g.e;
+  delete b;
+  return 0;
+}
+
+#define BATCH_RUN(b, run)                                                          \
+  if (!(b)) return fail(E_INVALID, "batch is NULL");                              \
+  if ((run) < 0 || (run) >= (int)(b)->where.size()) return fail(E_INVALID, "run out of range"); \
+  auto& G_ = (b)->groups[(b)->where[(run)].first];                                 \
+  const int L_ = (b)->where[(run)].second;
+
+int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields) {
+  BATCH_RUN(b, run);
+  return G_.e->set_ic_real(L_, fields);
+}
+int32_t mhd_batch_set_sol(mhd_batch* b, int32_t run, const void* host_in) {
+  BATCH_RUN(b, run);
+  return G_.e->set_sol(L_, host_in);
+}
+int32_t mhd_batch_get_sol(mhd_batch* b, int32_t run, void* host_out) {
+  BATCH_RUN(b, run);
+  return G_.e->get_sol(L_, host_out);
+}
+int32_t mhd_batch_get_clock(mhd_batch* b, int32_t run, double* t, double* dt, int64_t* step) {
+  BATCH_RUN(b, run);
+  return G_.e->get_clock(L_, t, dt, step);
+}
+int32_t mhd_batch_set_cfl(mhd_batch* b, int32_t run, double cfl_coef) {
+  BATCH_RUN(b, run);
+  return G_.e->set_cfl(L_, cfl_coef > 0 ? 1 : 0, cfl_coef);
+}
+int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
+  // every group steps on its own stream; the groups overlap on the device
+  for (auto& g : b->groups) MHD_TRY(g.e->step(nsteps));
+  for (auto& g : b->groups) MHD_TRY(g.e->sync());
+  return 0;
+}
+int32_t mhd_batch_use_graph(mhd_batch* b, int32_t enable) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
+  for (auto& g : b->groups) MHD_TRY(g.e->set_graph(enable));
+  return 0;
+}
+int32_t mhd_batch_time_steps(mhd_batch* b, int32_t nsteps, double* ms_out) {
+  if (!b || !ms_out) return fail(E_INVALID, "batch or ms_out is NULL");
+  // warm the graphs, then time all groups' streams together
+  for (auto& g : b->groups) MHD_TRY(g.e->step(0));
+  for (auto& g : b->groups) MHD_TRY(g.e->sync());
+  hipEvent_t a, e;
+  MHD_HIP(hipEventCreate(&a));
+  MHD_HIP(hipEventCreate(&e));
+  hipStream_t s0 = b->groups[0].e->stream();
+  MHD_HIP(hipEventRecord(a, s0));
+  for (size_t i = 1; i < b->groups.size(); ++i) MHD_HIP(hipStreamWaitEvent(b->groups[i].e->stream(), a, 0));
+  for (auto& g : b->groups) MHD_TRY(g.e->step(nsteps));
+  std::vector<hipEvent_t> done(b->groups.size());
+  for (size_t i = 1; i < b->groups.size(); ++i) {
+    MHD_HIP(hipEventCreate(&done[i]));
+    MHD_HIP(hipEventRecord(done[i], b->groups[i].e->stream()));
+    MHD_HIP(hipStreamWaitEvent(s0, done[i], 0));
+  }
+  MHD_HIP(hipEventRecord(e, s0));
+  MHD_HIP(hipEventSynchronize(e));
+  float ms = 0;
+  MHD_HIP(hipEventElapsedTime(&ms, a, e));
+  for (size_t i = 1; i < done.size(); ++i) hipEventDestroy(done[i]);
+  hipEventDestroy(a);
+  hipEventDestroy(e);
+  *ms_out = ms;
+  return 0;
+}
+
+}  // extern "C"

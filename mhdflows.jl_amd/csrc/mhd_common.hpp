@@ -1,0 +1,99 @@
+// Shared definitions of the MI355X MHD library: grid geometry, per-run buffer
+// descriptors and the FourierFlows 1/3 box-mask arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+#include "mhd_fft.hpp"
+
+namespace mhd {
+
+enum Phys { PHYS_HD = 0, PHYS_MHD = 1 };
+enum Stepper { STEP_RK4 = 0, STEP_LSRK54 = 1 };
+
+// Buffer selectors inside a RunBuf.
+enum BufSel { B_SOL = 0, B_STIN = 1, B_ACC = 2, B_WORK = 3, B_EXT_SOL = 4, B_EXT_N = 5, B_SCR = 6, NBUF = 7 };
+
+// Geometry of one grid size.  Arrays are [f][z][y][kx] (complex, nkr = nx/2+1)
+// and [f][z][y][x] (real), i.e. Julia's column-major (kr,l,m,f) / (x,y,z).
+struct Geom {
+  int nx, ny, nz, nkr;
+  // kept index sets of the 1/3 box mask: x keeps kx < kx_keep; y keeps
+  // l < ly_lo_end or l >= ly_hi_begin; z likewise.
+  int kx_keep, ly_lo_end, ly_hi_begin, mz_lo_end, mz_hi_begin;
+  int64_t nc;      // complex elements per field  = nz*ny*nkr
+  int64_t nr;      // real elements per field     = nz*ny*nx
+  double multx, multy, multz;  // wavenumber multipliers ((2pi/L)*n)/n
+  double inv_n;    // 1/(nx*ny*nz) in Float64 (rounded to T by the kernels)
+};
+
+// FourierFlows getaliasedwavenumbers with aliased fraction 1/3 (as called at
+// src/utils/utils.jl:82-84): L=(1-a)/2, R=(1+a)/2, iL=floor(L n)+1, iR=ceil(R n)
+// (1-based), aliased = iL:iR (full axes) and iL:nkr (half axis).  Returns the
+// 0-based [lo, hi) aliased block.
+inline void aliased_block(int n, int* lo, int* hi) {
+  const double a = 1.0 / 3.0;
+  const double L = (1.0 - a) / 2.0, R = (1.0 + a) / 2.0;
+  const int iL = (int)std::floor(L * n) + 1;
+  const int iR = (int)std::ceil(R * n);
+  *lo = iL - 1;
+  *hi = iR;
+}
+
+inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
+  Geom g;
+  g.nx = nx; g.ny = ny; g.nz = nz; g.nkr = nx / 2 + 1;
+  int lo, hi;
+  aliased_block(nx, &lo, &hi);
+  g.kx_keep = lo;
+  aliased_block(ny, &lo, &hi);
+  g.ly_lo_end = lo; g.ly_hi_begin = hi;
+  aliased_block(nz, &lo, &hi);
+  g.mz_lo_end = lo; g.mz_hi_begin = hi;
+  g.nc = (int64_t)nz * ny * g.nkr;
+  g.nr = (int64_t)nz * ny * nx;
+  const double twopi = 6.283185307179586;  // Julia's 2π as Float64
+  g.multx = ((twopi / Lx) * nx) / nx;
+  g.multy = ((twopi / Ly) * ny) / ny;
+  g.multz = ((twopi / Lz) * nz) / nz;
+  g.inv_n = 1.0 / ((double)nx * (double)ny * (double)nz);
+  return g;
+}
+
+__host__ __device__ inline bool kept_y(const Geom& g, int l) { return l < g.ly_lo_end || l >= g.ly_hi_begin; }
+__host__ __device__ inline bool kept_z(const Geom& g, int m) { return m < g.mz_lo_end || m >= g.mz_hi_begin; }
+__host__ __device__ inline int nkept_y(const Geom& g) { return g.ly_lo_end + g.ny - g.ly_hi_begin; }
+__host__ __device__ inline int nkept_z(const Geom& g) { return g.mz_lo_end + g.nz - g.mz_hi_begin; }
+__host__ __device__ inline int signed_index(int i, int n) { return i < (n + 1) / 2 ? i : i - n; }
+
+// Device-side clock of one run.  dt is chosen on the device (CFL) so a whole step
+// can be replayed from a captured graph without a host round trip.
+struct RunClock {
+  double t;          // clock.t (holds a value representable in T)
+  double dt;         // clock.dt (representable in T)
+  int64_t step;      // clock.step
+  double maxu2;      // max_a max_x u_a^2 of the current vars (T value)
+  double maxb2;      // max_a max_x b_a^2 of the current vars (T value)
+  double cfl_coef;   // CFL coefficient
+  double t_diff;     // diffusive bound (Float64)
+  double dl;         // min(dx,dy,dz) in T
+  int32_t use_cfl;   // 1: dt from getCFL! before each step
+  int32_t nan_flag;  // set when the maxima are NaN
+};
+
+// Per-run descriptor read by every kernel (one per run of a batched launch).
+template <class T>
+struct RunBuf {
+  cx<T>* buf[NBUF];   // complex buffers by BufSel (nf or np fields each)
+  T* vars;            // nf real fields
+  T* rscr;            // real scratch (>= 3 real fields), aliases the stage-input buffer
+  T* partial;         // per-workgroup maxima (2 per row block), CFL input
+  RunClock* clk;
+  const cx<T>* force; // forcing table (3 comps x (K+1) x (2K+1)^2) or null
+  int force_K;
+  int nnu, neta;
+  double nu, eta;
+};
+
+}  // namespace mhd

@@ -1,0 +1,883 @@
+// Engine<T>: launch sequences of the MHD/HD right-hand side and time steps for a
+// group of runs sharing one grid size and precision.
+#include "mhd_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace mhd {
+
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables, generated in extended precision and rounded to T
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tab_mu;
+std::map<std::tuple<int, int, int, int>, void*> g_tabs;  // (device, sizeof(T), n, kind) -> device table
+
+template <class T>
+const cx<T>* twiddles(int device, int n, int kind, int* err) {
+  // kind 0: tw[m] = exp(-2 pi i m / n), m < n ; kind 1: exp(-2 pi i k / n), k <= n/2
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto key = std::make_tuple(device, (int)sizeof(T), n, kind);
+  auto it = g_tabs.find(key);
+  if (it != g_tabs.end()) return (const cx<T>*)it->second;
+  const int cnt = kind == 0 ? n : n / 2 + 1;
+  std::vector<cx<T>> h(cnt);
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (int m = 0; m < cnt; ++m) {
+    // reduce to the first octant so the long-double sin/cos see small arguments
+    long double ang = 2.0L * pi * (long double)m / (long double)n;
+    h[m].re = (T)cosl(ang);
+    h[m].im = (T)(-sinl(ang));
+    // exact values at the quarter points
+    if (4 * (int64_t)m % n == 0) {
+      const int q = (int)(4 * (int64_t)m / n) % 4;
+      const T c[4] = {T(1), T(0), T(-1), T(0)}, s[4] = {T(0), T(-1), T(0), T(1)};
+      h[m].re = c[q];
+      h[m].im = s[q];
+    }
+  }
+  void* d = nullptr;
+  if (hipMalloc(&d, cnt * sizeof(cx<T>)) != hipSuccess) {
+    *err = fail(E_HIP, "hipMalloc of twiddle table failed");
+    return nullptr;
+  }
+  if (hipMemcpy(d, h.data(), cnt * sizeof(cx<T>), hipMemcpyHostToDevice) != hipSuccess) {
+    *err = fail(E_HIP, "hipMemcpy of twiddle table failed");
+    return nullptr;
+  }
+  g_tabs[key] = d;
+  return (const cx<T>*)d;
+}
+
+// splitmix64-based uniform in [0,1), as the forcing definition in the oracle
+uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uniform01(uint64_t seed, uint64_t counter) {
+  const uint64_t x = seed * 0x9E3779B97F4A7C15ull + counter + 1;
+  return (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0);
+}
+}  // namespace
+
+int64_t run_bytes(const Geom& g, int phys, int prec) {
+  const int s = prec == 0 ? 4 : 8;
+  const int nf = phys == PHYS_MHD ? 6 : 3;
+  const int np = phys == PHYS_MHD ? 9 : 6;
+  const int64_t rb = (int64_t)row_rows_per_block(g.nx);
+  const int64_t blocks = ((int64_t)g.ny * g.nz + rb - 1) / rb;
+  return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s + 4096;
+}
+
+template <class T>
+class Engine : public EngineBase {
+ public:
+  Geom g{};
+  int phys = 0, nf = 0, np = 0, stepper = 0, nr_ = 0, dev = 0;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  std::vector<RunBuf<T>> rbh;
+  RunBuf<T>* rbd = nullptr;
+  std::vector<void*> allocs;
+  std::vector<RunClock> clk_init;
+  const cx<T>*twy = nullptr, *twz = nullptr, *twM = nullptr, *twh = nullptr;
+  int64_t row_blocks = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool use_graph = true;
+  bool profiling = false;
+  struct Ev {
+    const char* name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Ev> evs;
+  std::vector<void*> force_tabs;
+  double* dpart = nullptr;     // FP64 diagnostic partials
+  int dpart_blocks = 0;
+
+  ~Engine() override {
+    if (st) hipStreamSynchronize(st);
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    for (void* p : allocs) hipFree(p);
+    for (void* p : force_tabs)
+      if (p) hipFree(p);
+    if (dpart) hipFree(dpart);
+    if (rbd) hipFree(rbd);
+    if (own_stream && st) hipStreamDestroy(st);
+  }
+
+  int nruns() const override { return nr_; }
+  hipStream_t stream() const override { return st; }
+  int sync() override {
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  int init(const Geom& geom, const double* L, int ph, int stp, const std::vector<RunParams>& rp, int device,
+           hipStream_t s) {
+    g = geom;
+    lens[0] = L[0];
+    lens[1] = L[1];
+    lens[2] = L[2];
+    phys = ph;
+    nf = ph == PHYS_MHD ? 6 : 3;
+    np = ph == PHYS_MHD ? 9 : 6;
+    stepper = stp;
+    nr_ = (int)rp.size();
+    dev = device;
+    MHD_HIP(hipSetDevice(device));
+    if (s) {
+      st = s;
+    } else {
+      MHD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      own_stream = true;
+    }
+    int err = 0;
+    twy = twiddles<T>(device, g.ny, 0, &err);
+    twz = twiddles<T>(device, g.nz, 0, &err);
+    twM = twiddles<T>(device, g.nx / 2, 0, &err);
+    twh = twiddles<T>(device, g.nx, 1, &err);
+    if (err) return err;
+    const int64_t rpb = row_rows_per_block(g.nx);
+    row_blocks = ((int64_t)g.ny * g.nz + rpb - 1) / rpb;
+    rbh.resize(nr_);
+    force_tabs.assign(nr_, nullptr);
+    for (int r = 0; r < nr_; ++r) {
+      RunBuf<T>& b = rbh[r];
+      std::memset(&b, 0, sizeof(b));
+      auto alloc = [&](size_t bytes, void** p) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) return fail(E_HIP, "hipMalloc failed (out of device memory?)");
+        allocs.push_back(*p);
+        return hipMemsetAsync(*p, 0, bytes, st) == hipSuccess ? 0 : fail(E_HIP, "hipMemsetAsync failed");
+      };
+      void* p;
+      const size_t cb = sizeof(cx<T>) * (size_t)g.nc;
+      MHD_TRY(alloc(cb * nf, &p));
+      b.buf[B_SOL] = (cx<T>*)p;
+      MHD_TRY(alloc(cb * nf, &p));
+      b.buf[B_STIN] = (cx<T>*)p;
+      b.rscr = (T*)p;
+      MHD_TRY(alloc(cb * nf, &p));
+      b.buf[B_ACC] = (cx<T>*)p;
+      MHD_TRY(alloc(cb * np, &p));
+      b.buf[B_WORK] = (cx<T>*)p;
+      MHD_TRY(alloc(sizeof(T) * (size_t)g.nr * nf, &p));
+      b.vars = (T*)p;
+      MHD_TRY(alloc(sizeof(T) * 2 * (size_t)row_blocks, &p));
+      b.partial = (T*)p;
+      MHD_TRY(alloc(sizeof(RunClock), &p));
+      b.clk = (RunClock*)p;
+      b.nu = rp[r].nu;
+      b.eta = rp[r].eta;
+      b.nnu = rp[r].nnu;
+      b.neta = rp[r].neta;
+      RunClock c;
+      std::memset(&c, 0, sizeof(c));
+      c.dt = (double)(T)rp[r].dt;
+      c.dl = (double)std::min({clk_dx(0), clk_dx(1), clk_dx(2)});
+      c.t_diff = std::numeric_limits<double>::infinity();
+      clk_init.push_back(c);
+      MHD_HIP(hipMemcpyAsync(b.clk, &c, sizeof(c), hipMemcpyHostToDevice, st));
+    }
+    MHD_HIP(hipMalloc(&rbd, sizeof(RunBuf<T>) * nr_));
+    MHD_TRY(upload_runs());
+    dpart_blocks = 1024;
+    MHD_HIP(hipMalloc(&dpart, sizeof(double) * 3 * dpart_blocks));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  int upload_runs() {
+    MHD_HIP(hipMemcpyAsync(rbd, rbh.data(), sizeof(RunBuf<T>) * nr_, hipMemcpyHostToDevice, st));
+    return 0;
+  }
+
+  // ---- launch helpers (optionally timed per kernel class) -------------------
+  template <class F>
+  int timed(const char* name, double bytes, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (profiling) {
+      MHD_HIP(hipEventCreate(&a));
+      MHD_HIP(hipEventCreate(&b));
+      MHD_HIP(hipEventRecord(a, st));
+    }
+    hipError_t e = f();
+    if (e != hipSuccess) return fail(E_HIP, std::string("launch of ") + name + " failed: " + hipGetErrorString(e));
+    if (profiling) {
+      MHD_HIP(hipEventRecord(b, st));
+      evs.push_back({name, a, b, bytes});
+    }
+    return 0;
+  }
+
+  static constexpr int S = strip_width<T>();
+
+  int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
+              bool prune) {
+    ColArgs a{};
+    a.g = g;
+    a.axis = axis;
+    a.nfields = nfields;
+    a.src = src;
+    a.dst = dst;
+    a.src_f0 = src_f0;
+    a.dst_f0 = dst_f0;
+    a.ncols = prune ? g.kx_keep : g.nkr;
+    a.nstrips = (a.ncols + S - 1) / S;
+    const int n = axis == 1 ? g.ny : g.nz;
+    if (axis == 2 && prune) {  // only pencils with kept l
+      a.nother = nkept_y(g);
+      a.other_lo_end = g.ly_lo_end;
+      a.other_gap = g.ly_hi_begin - g.ly_lo_end;
+    } else {
+      a.nother = axis == 1 ? g.nz : g.ny;
+      a.other_lo_end = a.nother;
+      a.other_gap = 0;
+    }
+    if (prune) {
+      a.mask_lo = axis == 1 ? g.ly_lo_end : g.mz_lo_end;
+      a.mask_hi = axis == 1 ? g.ly_hi_begin : g.mz_hi_begin;
+    } else {
+      a.mask_lo = a.mask_hi = 0;
+    }
+    a.nruns = nr_;
+    a.nitems = (int64_t)nr_ * nfields * a.nother * a.nstrips;
+    const double cxb = sizeof(cx<T>);
+    const double cols = (double)nr_ * nfields * a.nother * a.ncols;
+    const double bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
+    const cx<T>* tw = axis == 1 ? twy : twz;
+    return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, rbd, tw, st); });
+  }
+
+  RowArgs row_args() const {
+    RowArgs r{};
+    r.g = g;
+    r.nruns = nr_;
+    r.rows = (int64_t)g.ny * g.nz;
+    r.blocks_per_run = row_blocks;
+    r.in_ncols = g.nkr;
+    return r;
+  }
+
+  int xrow(const char* name, const RowArgs& a, int kind, int nfk, double bytes) {
+    return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, rbd, twM, twh, st); });
+  }
+
+  // ---- one right-hand side evaluation ----------------------------------------
+  // in_sel: stage input; prune: the stage input is dealiased; last: write vars + maxima.
+  int rhs(int in_sel, bool prune, SpecArgs sp, bool last) {
+    MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune));
+    RowArgs ra = row_args();
+    ra.in_sel = B_WORK;
+    ra.in_ncols = prune ? g.kx_keep : g.nkr;
+    ra.out_sel = B_WORK;
+    ra.write_vars = last ? 1 : 0;
+    ra.want_max = last ? 1 : 0;
+    const double rows = (double)nr_ * g.ny * g.nz;
+    double bytes = rows * (nf * ra.in_ncols + np * g.nkr) * sizeof(cx<T>) + (last ? rows * nf * g.nx * sizeof(T) : 0);
+    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false));
+    sp.g = g;
+    sp.nruns = nr_;
+    sp.nitems = (int64_t)nr_ * g.nc;
+    sp.prod_sel = B_WORK;
+    MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
+    if (last) MHD_TRY(timed("reduce", 0, [&] {
+      return launch_small<T>(SK_REDUCE, rbd, nr_, phys, row_blocks, sp.mode == SP_RHS ? 0 : 1, st);
+    }));
+    return 0;
+  }
+
+  double spectral_bytes(const SpecArgs& sp) const {
+    const double c = sizeof(cx<T>) * (double)nr_;
+    const double kept = (double)g.kx_keep * nkept_y(g) * nkept_z(g);
+    const double all = (double)g.nc;
+    double rd = np * all, wr = 0;
+    if (sp.mode == SP_RK4) {
+      rd += nf * kept;                                  // stage input
+      if (sp.stage > 0) rd += nf * kept + nf * all;     // base sol + acc
+      wr += sp.stage < 3 ? nf * all + nf * kept : nf * all;
+    } else if (sp.mode == SP_LSRK54) {
+      rd += nf * kept + (sp.stage > 0 ? nf * all : 0);
+      wr += 2 * nf * all;
+    } else {
+      rd += nf * (sp.self_sel >= 0 ? all : kept);
+      wr += (sp.f_hi - sp.f_lo) * all;
+    }
+    return (rd + wr) * c;
+  }
+
+  SpecArgs spec_base() const {
+    SpecArgs s{};
+    s.g = g;
+    s.mask_input = 1;
+    s.add_force = 1;
+    s.self_sel = -1;
+    s.f_lo = 0;
+    s.f_hi = nf;
+    return s;
+  }
+
+  // ---- one time step -----------------------------------------------------------
+  int step_launches() {
+    MHD_TRY(timed("cfl", 0, [&] { return launch_small<T>(SK_CFL, rbd, nr_, phys, 0, 0, st); }));
+    if (stepper == STEP_RK4) {
+      for (int s = 0; s < 4; ++s) {
+        SpecArgs sp = spec_base();
+        sp.mode = SP_RK4;
+        sp.stage = s;
+        sp.in_sel = s == 0 ? B_SOL : B_STIN;
+        MHD_TRY(rhs(sp.in_sel, true, sp, s == 3));
+      }
+    } else {
+      static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                  -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+      static const double B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                  1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                  2277821191437.0 / 14882151754819.0};
+      for (int s = 0; s < 5; ++s) {
+        SpecArgs sp = spec_base();
+        sp.mode = SP_LSRK54;
+        sp.stage = s;
+        sp.in_sel = B_SOL;
+        sp.ca = A[s];
+        sp.cb = B[s];
+        MHD_TRY(rhs(B_SOL, true, sp, s == 4));
+      }
+    }
+    return 0;
+  }
+
+  int ensure_graph() {
+    if (gexec) return 0;
+    MHD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = step_launches();
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &gr);
+    if (rc) {
+      if (gr) hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    graph = gr;
+    MHD_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    return 0;
+  }
+
+  int step(int nsteps) override {
+    if (nsteps < 0) return fail(E_INVALID, "nsteps must be >= 0");
+    if (use_graph && !profiling) {
+      MHD_TRY(ensure_graph());
+      for (int i = 0; i < nsteps; ++i) MHD_HIP(hipGraphLaunch(gexec, st));
+      return 0;
+    }
+    for (int i = 0; i < nsteps; ++i) MHD_TRY(step_launches());
+    return 0;
+  }
+
+  int set_graph(int enable) override {
+    use_graph = enable != 0;
+    return 0;
+  }
+
+  // ---- state boundary -------------------------------------------------------------
+  int check_run(int run) const {
+    if (run < 0 || run >= nr_) return fail(E_INVALID, "run index out of range");
+    return 0;
+  }
+
+  int set_ic_real(int run, const void* const* fields) override {
+    MHD_TRY(check_run(run));
+    RunBuf<T>& b = rbh[run];
+    bool any = false;
+    for (int f = 0; f < nf; ++f) {
+      if (!fields[f]) continue;
+      any = true;
+      MHD_HIP(hipMemcpyAsync(b.vars + (int64_t)f * g.nr, fields[f], sizeof(T) * g.nr, hipMemcpyHostToDevice, st));
+    }
+    if (!any) return 0;
+    MHD_TRY(r2c_vars_to_work(run));
+    for (int f = 0; f < nf; ++f) {
+      if (!fields[f]) continue;
+      MHD_HIP(hipMemcpyAsync(b.buf[B_SOL] + (int64_t)f * g.nc, b.buf[B_WORK] + (int64_t)f * g.nc,
+                             sizeof(cx<T>) * g.nc, hipMemcpyDeviceToDevice, st));
+    }
+    MHD_TRY(vars_maxima());
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  // R2C of all vars fields into work[0..nf) (every run of the group)
+  int r2c_vars_to_work(int /*run*/) {
+    for (int f0 = 0; f0 < nf; f0 += 3) {
+      RowArgs ra = row_args();
+      ra.in_f0 = f0;
+      ra.real_sel = 0;
+      ra.out2_sel = B_WORK;
+      ra.out2_f0 = f0;
+      MHD_TRY(xrow("xrow_r2c", ra, ROW_R2C, 3, 0));
+    }
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, nf, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, nf, false));
+    return 0;
+  }
+
+  // max u^2 / b^2 over vars into the clocks (getCFL! input after the fields are set)
+  int vars_maxima() {
+    const int64_t nb = std::min<int64_t>(row_blocks, 1024);
+    MHD_TRY(timed("varsmax", 0, [&] { return launch_small<T>(SK_VARSMAX, rbd, nr_, nf, nb, (int)g.nr, st); }));
+    MHD_TRY(timed("reduce", 0, [&] { return launch_small<T>(SK_REDUCE, rbd, nr_, phys, nb, 0, st); }));
+    return 0;
+  }
+
+  int get_sol(int run, void* host) override {
+    MHD_TRY(check_run(run));
+    MHD_HIP(hipMemcpyAsync(host, rbh[run].buf[B_SOL], sizeof(cx<T>) * g.nc * nf, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int set_sol(int run, const void* host) override {
+    MHD_TRY(check_run(run));
+    MHD_HIP(hipMemcpyAsync(rbh[run].buf[B_SOL], host, sizeof(cx<T>) * g.nc * nf, hipMemcpyHostToDevice, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int get_vars(int run, int f, void* host) override {
+    MHD_TRY(check_run(run));
+    if (f < 0 || f >= nf) return fail(E_INVALID, "field index out of range");
+    MHD_HIP(hipMemcpyAsync(host, rbh[run].vars + (int64_t)f * g.nr, sizeof(T) * g.nr, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int device_ptrs(int run, void** sol, void** vars) override {
+    MHD_TRY(check_run(run));
+    if (sol) *sol = rbh[run].buf[B_SOL];
+    if (vars) *vars = rbh[run].vars;
+    return 0;
+  }
+  int get_clock(int run, double* t, double* dt, int64_t* step) override {
+    MHD_TRY(check_run(run));
+    RunClock c;
+    MHD_HIP(hipMemcpyAsync(&c, rbh[run].clk, sizeof(c), hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    if (t) *t = c.t;
+    if (dt) *dt = c.dt;
+    if (step) *step = c.step;
+    return 0;
+  }
+  int read_clock(int run, RunClock* c) {
+    MHD_HIP(hipMemcpyAsync(c, rbh[run].clk, sizeof(RunClock), hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int write_clock(int run, const RunClock& c) {
+    MHD_HIP(hipMemcpyAsync(rbh[run].clk, &c, sizeof(RunClock), hipMemcpyHostToDevice, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int set_clock(int run, double t, double dt, int64_t step) override {
+    MHD_TRY(check_run(run));
+    RunClock c;
+    MHD_TRY(read_clock(run, &c));
+    c.t = (double)(T)t;
+    c.dt = (double)(T)dt;
+    c.step = step;
+    return write_clock(run, c);
+  }
+  int set_cfl(int run, int use, double coef) override {
+    MHD_TRY(check_run(run));
+    RunClock c;
+    MHD_TRY(read_clock(run, &c));
+    c.use_cfl = use;
+    c.cfl_coef = coef;
+    c.t_diff = t_diff(run, coef);
+    return write_clock(run, c);
+  }
+
+  // diffusive bound t_diff (integrator.jl:60-72); nη is never forwarded, so it is 0
+  double t_diff(int run, double coef) const {
+    const RunBuf<T>& b = rbh[run];
+    double vi, nv;
+    if (phys == PHYS_MHD) {
+      vi = std::max(b.nu, b.eta);
+      nv = std::max(b.nnu, b.neta);
+    } else {
+      vi = b.nu;
+      nv = b.nnu;
+    }
+    // dl^nv is evaluated in T before the Float64 coefficient multiplies it
+    const T dl = (T)clk_init[run].dl;
+    if (vi == 0) return std::numeric_limits<double>::infinity();
+    T p = dl;
+    for (int i = 1; i < (nv > 1 ? (int)nv : 2); ++i) p = p * dl;
+    return coef * (double)p / vi;
+  }
+
+  // ---- operator boundary -------------------------------------------------------
+  int calcN(void* N, void* sol, int advection_only) override {
+    if (nr_ != 1) return fail(E_INVALID, "calcN acts on single-run problems");
+    rbh[0].buf[B_EXT_SOL] = (cx<T>*)sol;
+    rbh[0].buf[B_EXT_N] = (cx<T>*)N;
+    MHD_TRY(upload_runs());
+    SpecArgs sp = spec_base();
+    sp.mode = SP_RHS;
+    sp.in_sel = B_EXT_SOL;
+    sp.out_sel = B_EXT_N;
+    sp.out_f0 = 0;
+    sp.mask_input = advection_only ? 0 : 1;
+    sp.zero_aliased = advection_only ? 0 : 1;
+    sp.add_force = advection_only ? 0 : 1;
+    MHD_TRY(rhs(B_EXT_SOL, !advection_only, sp, true));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  int component_update(void* N, int field) override {
+    if (nr_ != 1) return fail(E_INVALID, "component updates act on single-run problems");
+    if (field < 0 || field >= nf) return fail(E_INVALID, "direction out of range");
+    rbh[0].buf[B_EXT_N] = (cx<T>*)N;
+    MHD_TRY(upload_runs());
+    RowArgs ra = row_args();
+    ra.in_f0 = 0;
+    ra.real_sel = 0;
+    ra.out_sel = B_WORK;
+    ra.out_f0 = 0;
+    ra.out2_sel = B_STIN;
+    ra.out2_f0 = 0;
+    MHD_TRY(xrow("xrow_vars", ra, phys == PHYS_MHD ? ROW_VARS_MHD : ROW_VARS_HD, nf, 0));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_STIN, B_STIN, 0, 0, nf, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_STIN, B_STIN, 0, 0, nf, false));
+    SpecArgs sp = spec_base();
+    sp.nruns = 1;
+    sp.nitems = g.nc;
+    sp.mode = SP_RHS;
+    sp.in_sel = B_STIN;
+    sp.mask_input = 0;
+    sp.add_force = 0;
+    sp.prod_sel = B_WORK;
+    sp.self_sel = B_STIN;
+    sp.self_f0 = 0;
+    sp.out_sel = B_EXT_N;
+    sp.out_f0 = field;
+    sp.f_lo = field;
+    sp.f_hi = field + 1;
+    MHD_TRY(timed("spectral", 0, [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  int cfl_now(double coef, double* dt) override {
+    for (int r = 0; r < nr_; ++r) MHD_TRY(set_cfl(r, 1, coef));
+    MHD_TRY(timed("cfl", 0, [&] { return launch_small<T>(SK_CFL, rbd, nr_, phys, 0, 0, st); }));
+    RunClock c;
+    MHD_TRY(read_clock(0, &c));
+    for (int r = 0; r < nr_; ++r) {
+      RunClock cr;
+      MHD_TRY(read_clock(r, &cr));
+      cr.use_cfl = 0;
+      MHD_TRY(write_clock(r, cr));
+    }
+    if (dt) *dt = c.dt;
+    return 0;
+  }
+
+  int integrate(double t0, int64_t N0, double usr_dt, double coef, int64_t* taken) override {
+    if (nr_ != 1) return fail(E_INVALID, "integrate acts on single-run problems");
+    RunClock c;
+    MHD_TRY(read_clock(0, &c));
+    c.step = 0;  // integrator.jl:76
+    if (usr_dt != 0.0) {
+      c.dt = (double)(T)usr_dt;
+      c.use_cfl = 0;
+    } else {
+      c.use_cfl = 1;
+      c.cfl_coef = coef;
+      c.t_diff = t_diff(0, coef);
+    }
+    MHD_TRY(write_clock(0, c));
+    int64_t n = 0;
+    if (std::isinf(t0) && t0 > 0) {
+      // while N0 >= step: exactly N0+1 steps, no host round trips
+      if (N0 >= 0) {
+        int64_t left = N0 + 1;
+        while (left > 0) {
+          const int chunk = (int)std::min<int64_t>(left, 1 << 20);
+          MHD_TRY(step(chunk));
+          left -= chunk;
+        }
+        n = N0 + 1;
+      }
+    } else {
+      while (true) {
+        MHD_TRY(read_clock(0, &c));
+        if (!(N0 >= c.step && t0 >= c.t)) break;
+        MHD_TRY(step(1));
+        ++n;
+      }
+    }
+    MHD_TRY(read_clock(0, &c));
+    c.use_cfl = 0;
+    MHD_TRY(write_clock(0, c));
+    if (taken) *taken = n;
+    return 0;
+  }
+
+  // ---- vars refresh and diagnostics -------------------------------------------
+  int refresh_vars() override {
+    MHD_TRY(colpass("zpass_inv", 2, 1, B_SOL, B_WORK, 0, 0, nf, true));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, true));
+    for (int f0 = 0; f0 < nf; f0 += 3) {
+      RowArgs ra = row_args();
+      ra.in_sel = B_WORK;
+      ra.in_f0 = f0;
+      ra.in_ncols = g.kx_keep;
+      ra.real_sel = 0;
+      ra.real_out_f0 = f0;
+      ra.write_vars = 1;
+      MHD_TRY(xrow("xrow_c2r", ra, ROW_C2R, 3, 0));
+    }
+    MHD_TRY(vars_maxima());
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  // spectral op on work[0..3) -> work[3..), back to real space into rscr[0..ncomp)
+  int diag_field(int op, int ncomp) {
+    for (int r = 0; r < nr_; ++r)
+      MHD_TRY(timed("diag", 0, [&] {
+        return launch_diag_spectral<T>(g, rbh[r].buf[B_WORK], rbh[r].buf[B_WORK] + 3 * g.nc, op, st);
+      }));
+    MHD_TRY(colpass("zpass_inv", 2, 1, B_WORK, B_WORK, 3, 3, ncomp, false));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 3, 3, ncomp, false));
+    RowArgs ra = row_args();
+    ra.in_sel = B_WORK;
+    ra.in_f0 = 3;
+    ra.in_ncols = g.nkr;
+    ra.real_sel = 1;
+    ra.real_out_f0 = 0;
+    ra.write_vars = 1;
+    MHD_TRY(xrow("xrow_c2r", ra, ROW_C2R, ncomp, 0));
+    return 0;
+  }
+
+  int reduce_dpart(int nblk, int width, std::vector<double>* sums) {
+    std::vector<double> h((size_t)nblk * width);
+    MHD_HIP(hipMemcpyAsync(h.data(), dpart, sizeof(double) * nblk * width, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    sums->assign(width, 0.0);
+    for (int i = 0; i < nblk; ++i)
+      for (int w = 0; w < width; ++w) {
+        if (w == 1 && width == 2) {
+          const double v = h[(size_t)i * width + w];
+          (*sums)[w] = (v > (*sums)[w] || v != v) ? v : (*sums)[w];
+        } else {
+          (*sums)[w] += h[(size_t)i * width + w];
+        }
+      }
+    return 0;
+  }
+
+  int diagnostics(double* out) override {
+    if (nr_ != 1) return fail(E_INVALID, "diagnostics act on single-run problems");
+    const RunBuf<T>& b = rbh[0];
+    const double dV = (double)(T)((T)(clk_dx(0))) * (double)(T)clk_dx(1) * (double)(T)clk_dx(2);
+    const int nblk = dpart_blocks;
+    std::vector<double> s;
+    MHD_TRY(timed("diag", 0, [&] { return launch_field_sums<T>(b.vars, g.nr, nf, dpart, nblk, st); }));
+    MHD_TRY(reduce_dpart(nblk, 3, &s));
+    out[0] = s[0] * dV;
+    out[1] = s[1] * dV;
+    out[4] = s[2] * dV;
+    // kinetic helicity and max |div u|
+    MHD_TRY(r2c_fields(0));
+    MHD_TRY(diag_field(OP_CURL, 3));
+    MHD_TRY(timed("diag", 0, [&] { return launch_dot_max<T>(b.rscr, b.vars, g.nr, 3, dpart, nblk, st); }));
+    MHD_TRY(reduce_dpart(nblk, 2, &s));
+    out[2] = s[0] * dV;
+    MHD_TRY(diag_field(OP_DIV, 1));
+    MHD_TRY(timed("diag", 0, [&] { return launch_dot_max<T>(b.rscr, nullptr, g.nr, 1, dpart, nblk, st); }));
+    MHD_TRY(reduce_dpart(nblk, 2, &s));
+    out[6] = s[1];
+    if (phys == PHYS_MHD) {
+      MHD_TRY(r2c_fields(3));
+      MHD_TRY(diag_field(OP_VECPOT, 3));
+      MHD_TRY(timed("diag", 0, [&] {
+        return launch_dot_max<T>(b.rscr, b.vars + 3 * g.nr, g.nr, 3, dpart, nblk, st);
+      }));
+      MHD_TRY(reduce_dpart(nblk, 2, &s));
+      out[3] = s[0] * dV;
+      MHD_TRY(diag_field(OP_DIV, 1));
+      MHD_TRY(timed("diag", 0, [&] { return launch_dot_max<T>(b.rscr, nullptr, g.nr, 1, dpart, nblk, st); }));
+      MHD_TRY(reduce_dpart(nblk, 2, &s));
+      out[5] = s[1];
+    } else {
+      out[3] = out[5] = 0.0;
+    }
+    out[7] = (out[0] != out[0] || out[1] != out[1]) ? 1.0 : 0.0;
+    return 0;
+  }
+  double lens[3] = {2 * M_PI, 2 * M_PI, 2 * M_PI};
+  T clk_dx(int axis) const {
+    const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+    return (T)((T)lens[axis] / (T)n);
+  }
+
+  // R2C of vars fields f0..f0+2 into work[0..3)
+  int r2c_fields(int f0) {
+    RowArgs ra = row_args();
+    ra.in_f0 = f0;
+    ra.real_sel = 0;
+    ra.out2_sel = B_WORK;
+    ra.out2_f0 = 0;
+    MHD_TRY(xrow("xrow_r2c", ra, ROW_R2C, 3, 0));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    return 0;
+  }
+
+  // ---- forcing -------------------------------------------------------------------
+  int set_forcing(int run, uint64_t seed, double kf, double amp) override {
+    MHD_TRY(check_run(run));
+    if (force_tabs[run]) {
+      MHD_HIP(hipStreamSynchronize(st));
+      hipFree(force_tabs[run]);
+      force_tabs[run] = nullptr;
+    }
+    rbh[run].force = nullptr;
+    rbh[run].force_K = 0;
+    if (amp != 0.0 && kf > 0) {
+      const int K = (int)std::ceil(kf + 0.5);
+      const int64_t per = (int64_t)(K + 1) * (2 * K + 1) * (2 * K + 1);
+      std::vector<cx<T>> tab(3 * per, cx<T>{T(0), T(0)});
+      const double lo2 = (kf - 0.5) * (kf - 0.5), hi2 = (kf + 0.5) * (kf + 0.5);
+      const double Ntot = (double)g.nx * g.ny * g.nz;
+      int64_t i = 0;
+      for (int r = -K; r <= K; ++r)
+        for (int q = -K; q <= K; ++q)
+          for (int p = -K; p <= K; ++p) {
+            const double s2 = (double)(p * p + q * q + r * r);
+            if (!(lo2 <= s2 && s2 < hi2)) continue;
+            if (!(p > 0 || (p == 0 && q > 0) || (p == 0 && q == 0 && r > 0))) continue;
+            double fr[3], fi[3];
+            for (int c = 0; c < 3; ++c) {
+              const double ph = 2 * M_PI * uniform01(seed, (uint64_t)(3 * i + c));
+              fr[c] = amp * std::cos(ph);
+              fi[c] = amp * std::sin(ph);
+            }
+            const double k[3] = {p * 2 * M_PI / lens[0], q * 2 * M_PI / lens[1], r * 2 * M_PI / lens[2]};
+            const double kk = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+            const double dr = (k[0] * fr[0] + k[1] * fr[1] + k[2] * fr[2]) / kk;
+            const double di = (k[0] * fi[0] + k[1] * fi[1] + k[2] * fi[2]) / kk;
+            for (int c = 0; c < 3; ++c) {
+              fr[c] -= k[c] * dr;
+              fi[c] -= k[c] * di;
+            }
+            for (int sg = 1; sg >= -1; sg -= 2) {
+              const int kx = sg * p, ky = sg * q, kz = sg * r;
+              if (kx < 0) continue;
+              const int64_t fidx = ((int64_t)(kz + K) * (2 * K + 1) + (ky + K)) * (K + 1) + kx;
+              for (int c = 0; c < 3; ++c)
+                tab[c * per + fidx] = cx<T>{(T)(Ntot * fr[c]), (T)(Ntot * (sg > 0 ? fi[c] : -fi[c]))};
+            }
+            ++i;
+          }
+      void* d = nullptr;
+      MHD_HIP(hipMalloc(&d, sizeof(cx<T>) * tab.size()));
+      MHD_HIP(hipMemcpy(d, tab.data(), sizeof(cx<T>) * tab.size(), hipMemcpyHostToDevice));
+      force_tabs[run] = d;
+      rbh[run].force = (const cx<T>*)d;
+      rbh[run].force_K = K;
+    }
+    MHD_TRY(upload_runs());
+    MHD_HIP(hipStreamSynchronize(st));
+    // the captured step reads the descriptor array, which stays at the same address
+    return 0;
+  }
+
+  // ---- measurement -------------------------------------------------------------
+  int time_steps(int nsteps, double* ms) override {
+    hipEvent_t a, b;
+    MHD_HIP(hipEventCreate(&a));
+    MHD_HIP(hipEventCreate(&b));
+    if (use_graph) MHD_TRY(ensure_graph());
+    MHD_HIP(hipStreamSynchronize(st));
+    MHD_HIP(hipEventRecord(a, st));
+    MHD_TRY(step(nsteps));
+    MHD_HIP(hipEventRecord(b, st));
+    MHD_HIP(hipEventSynchronize(b));
+    float f = 0;
+    MHD_HIP(hipEventElapsedTime(&f, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = f;
+    return 0;
+  }
+
+  int profile_steps(int nsteps, std::vector<KTime>* out) override {
+    MHD_HIP(hipStreamSynchronize(st));
+    profiling = true;
+    evs.clear();
+    int rc = 0;
+    for (int i = 0; i < nsteps && rc == 0; ++i) rc = step_launches();
+    profiling = false;
+    if (rc) return rc;
+    MHD_HIP(hipStreamSynchronize(st));
+    std::map<std::string, KTime> agg;
+    std::vector<std::string> order;
+    for (auto& e : evs) {
+      float ms = 0;
+      MHD_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+      if (!agg.count(e.name)) order.push_back(e.name);
+      KTime& k = agg[e.name];
+      k.name = e.name;
+      k.launches += 1;
+      k.ms += ms;
+      k.bytes += e.bytes;
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+    evs.clear();
+    out->clear();
+    for (auto& n : order) out->push_back(agg[n]);
+    return 0;
+  }
+};
+
+EngineBase* make_engine(int prec, const Geom& g, const double* L, int phys, int stepper,
+                        const std::vector<RunParams>& runs, int device, hipStream_t stream, int* err) {
+  if (prec == 0) {
+    auto* e = new Engine<float>();
+    *err = e->init(g, L, phys, stepper, runs, device, stream);
+    if (*err) {
+      delete e;
+      return nullptr;
+    }
+    return e;
+  }
+  auto* e = new Engine<double>();
+  *err = e->init(g, L, phys, stepper, runs, device, stream);
+  if (*err) {
+    delete e;
+    return nullptr;
+  }
+  return e;
+}
+
+}  // namespace mhd

@@ -1,0 +1,79 @@
+// Host-side engine: buffers, twiddle tables and launch sequences of one group of
+// same-size runs (one run for a plain problem, many for an ensemble group).
+#pragma once
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "mhd_launch.hpp"
+
+namespace mhd {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+
+// error codes (mirror include/mhdflows_amd.h)
+constexpr int E_OK = 0, E_INVALID = -1, E_UNSUPPORTED = -2, E_BUDGET = -3, E_HIP = -4, E_NAN = -5;
+
+#define MHD_HIP(x)                                                                                   \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) return ::mhd::fail(::mhd::E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define MHD_TRY(x)             \
+  do {                         \
+    int r_ = (x);              \
+    if (r_ != 0) return r_;    \
+  } while (0)
+
+// Per-run physical parameters.
+struct RunParams {
+  double nu = 0, eta = 0;
+  int nnu = 0, neta = 0;
+  double dt = 0;
+};
+
+struct KTime {
+  std::string name;
+  int64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;
+};
+
+class EngineBase {
+ public:
+  virtual ~EngineBase() {}
+  virtual int nruns() const = 0;
+  virtual int set_ic_real(int run, const void* const* fields) = 0;
+  virtual int get_sol(int run, void* host) = 0;
+  virtual int set_sol(int run, const void* host) = 0;
+  virtual int get_vars(int run, int f, void* host) = 0;
+  virtual int device_ptrs(int run, void** sol, void** vars) = 0;
+  virtual int get_clock(int run, double* t, double* dt, int64_t* step) = 0;
+  virtual int set_clock(int run, double t, double dt, int64_t step) = 0;
+  virtual int set_cfl(int run, int use, double coef) = 0;
+  virtual int calcN(void* N, void* sol, int advection_only) = 0;
+  virtual int component_update(void* N, int field) = 0;
+  virtual int step(int nsteps) = 0;
+  virtual int cfl_now(double coef, double* dt) = 0;
+  virtual int integrate(double t0, int64_t N0, double usr_dt, double coef, int64_t* taken) = 0;
+  virtual int set_graph(int enable) = 0;
+  virtual int diagnostics(double* out8) = 0;
+  virtual int refresh_vars() = 0;
+  virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;
+  virtual int time_steps(int nsteps, double* ms) = 0;
+  virtual int profile_steps(int nsteps, std::vector<KTime>* out) = 0;
+  virtual int sync() = 0;
+  virtual hipStream_t stream() const = 0;
+};
+
+// Device bytes of one run.
+int64_t run_bytes(const Geom& g, int phys, int prec);
+
+EngineBase* make_engine(int prec, const Geom& g, const double* L, int phys, int stepper,
+                        const std::vector<RunParams>& runs, int device, hipStream_t stream, int* err);
+
+}  // namespace mhd

@@ -1,0 +1,777 @@
+// Kernel templates of the MI355X MHD right-hand side and time step.
+//
+// One right-hand-side evaluation (MHDcalcN!, src/pgen.jl:153-162) is six launches:
+//   colpass z (inverse)  stage input  -> work[0..nf)   only kept (kx, l) pencils, kept m rows read
+//   colpass y (inverse)  work -> work                  only kx < kx_keep columns, kept l rows read
+//   xrow (fused)         work[0..nf) -> work[0..np)    c2r, products u(x)u-b(x)b and u x b, r2c
+//   colpass y (forward)  work -> work                  all columns
+//   colpass z (forward)  work -> work                  all columns
+//   spectral             products + stage input -> RK update (or N)
+// Masked (aliased) modes of the stage input are exactly zero, so the inverse passes
+// never read them; every forward output mode is produced, as the reference
+// computes the right-hand side on the whole half spectrum.
+#pragma once
+#include "mhd_common.hpp"
+
+namespace mhd {
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs in
+// turn; remapping gives each XCD a contiguous range of items, so neighbouring
+// column strips (which share cache lines when rows are not 128 B aligned) land
+// in the same L2.
+__device__ __forceinline__ int64_t xcd_item(int64_t b, int64_t nb) {
+  const int64_t per = (nb + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
+// ---------------------------------------------------------------------------
+// column pass: FFT along y or z of S adjacent kx columns
+// ---------------------------------------------------------------------------
+struct ColArgs {
+  Geom g;
+  int axis;            // 1: y, 2: z
+  int nfields;         // fields per run
+  int src, dst;        // BufSel
+  int src_f0, dst_f0;  // first field index inside src / dst
+  int ncols;           // kx columns processed [0, ncols)
+  int nstrips;         // ceil(ncols / S)
+  int nother;          // other-axis indices processed
+  int other_lo_end;    // pruned other set: o < lo_end -> o, else o + other_gap
+  int other_gap;
+  int mask_lo, mask_hi;  // input positions in [mask_lo, mask_hi) are zero (not read)
+  int nruns;
+  int64_t nitems;
+};
+
+template <class T, int N, int S, int DIR>
+__global__ void __launch_bounds__(S*(N / Plan<N>::E))
+colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E;
+  constexpr int TH = N / E;
+  using Rad = typename Plan<N>::R;
+  constexpr int R0 = RFirst<Rad>::value;
+  constexpr int RL = RLast<Rad>::value;
+  __shared__ cx<T> lds[N * S];
+
+  const int t = threadIdx.x;
+  const int c = t % S;
+  const int g = t / S;
+  const Geom& G = a.g;
+  const int64_t estride = a.axis == 1 ? (int64_t)G.nkr : (int64_t)G.ny * G.nkr;
+  const int64_t ostride = a.axis == 1 ? (int64_t)G.ny * G.nkr : (int64_t)G.nkr;
+  auto ldsf = [&](int pos) -> cx<T>& { return lds[pos * S + c]; };
+
+  const int64_t nb = (int64_t)gridDim.x;
+  const int64_t item = xcd_item(blockIdx.x, nb);
+  if (item >= a.nitems) return;
+  int64_t it = item;
+  const int strip = (int)(it % a.nstrips); it /= a.nstrips;
+  const int oi = (int)(it % a.nother); it /= a.nother;
+  const int f = (int)(it % a.nfields);
+  const int run = (int)(it / a.nfields);
+  const int o = oi < a.other_lo_end ? oi : oi + a.other_gap;
+  const RunBuf<T>& rb = runs[run];
+  const int col = strip * S + c;
+  const bool col_ok = col < a.ncols;
+  const int64_t base = (int64_t)o * ostride + col;
+  const cx<T>* __restrict__ src = rb.buf[a.src] + (int64_t)(a.src_f0 + f) * G.nc + base;
+  cx<T>* __restrict__ dst = rb.buf[a.dst] + (int64_t)(a.dst_f0 + f) * G.nc + base;
+
+  cx<T> x[E];
+#pragma unroll
+  for (int i = 0; i < E / R0; ++i) {
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int p = g + i * TH + r * (N / R0);
+      const bool ok = col_ok && !(p >= a.mask_lo && p < a.mask_hi);
+      x[i * R0 + r] = ok ? src[p * estride] : cx<T>{T(0), T(0)};
+    }
+  }
+  fft_regs<T, N, E, DIR, Rad>(x, g, ldsf, tw);
+  if (col_ok) {
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int p = g + i * TH + r * (N / RL);
+        dst[p * estride] = x[i * RL + r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row kernel: transforms along x of whole rows, with the real-space work fused
+// ---------------------------------------------------------------------------
+enum RowKind {
+  ROW_MHD = 0,        // 6 complex rows -> c2r -> 9 products -> r2c
+  ROW_HD = 1,         // 3 complex rows -> c2r -> 6 products -> r2c
+  ROW_C2R = 2,        // NF complex rows -> NF real rows
+  ROW_R2C = 3,        // NF real rows -> NF complex rows
+  ROW_VARS_MHD = 4,   // 6 real rows -> 9 products and the 6 inputs -> r2c (15 rows)
+  ROW_VARS_HD = 5,    // 3 real rows -> 6 products and the 3 inputs -> r2c (9 rows)
+};
+
+struct RowArgs {
+  Geom g;
+  int nruns;
+  int in_sel;         // BufSel of the complex input (c2r kinds)
+  int in_f0;          // first input field (complex input, or real input rows)
+  int real_sel;       // real rows: 0 = vars, 1 = real scratch (rscr)
+  int in_ncols;       // kx entries read per input row (others are zero)
+  int out_sel;        // BufSel of the product spectra
+  int out_f0;         // first product field
+  int out2_sel;       // BufSel of the r2c of the inputs themselves (ROW_R2C / ROW_VARS_*)
+  int out2_f0;        // first field of those
+  int real_out_f0;    // c2r kinds: first real field written
+  int write_vars;     // c2r kinds: store the real rows into vars
+  int want_max;       // c2r kinds: per-block max of u^2 and b^2 into partial
+  int64_t rows;       // ny*nz
+  int64_t blocks_per_run;
+};
+
+// inputs, products and self-transforms (r2c of the inputs) of each row kind
+template <int KIND, int NF> struct RowKindInfo;
+template <int NF> struct RowKindInfo<ROW_MHD, NF> { static constexpr int NIN = 6, NPROD = 9, NSELF = 0; };
+template <int NF> struct RowKindInfo<ROW_HD, NF> { static constexpr int NIN = 3, NPROD = 6, NSELF = 0; };
+template <int NF> struct RowKindInfo<ROW_C2R, NF> { static constexpr int NIN = NF, NPROD = 0, NSELF = 0; };
+template <int NF> struct RowKindInfo<ROW_R2C, NF> { static constexpr int NIN = NF, NPROD = 0, NSELF = NF; };
+template <int NF> struct RowKindInfo<ROW_VARS_MHD, NF> { static constexpr int NIN = 6, NPROD = 9, NSELF = 6; };
+template <int NF> struct RowKindInfo<ROW_VARS_HD, NF> { static constexpr int NIN = 3, NPROD = 6, NSELF = 3; };
+
+// Real-space products of one point; u = v[0..2], b = v[3..5].
+// MHD: T_ij = b_i b_j - u_i u_j (i<=j: 00,01,02,11,12,22), M_aj = u_a b_j - b_a u_j (01,02,12).
+// HD:  U_ij = u_i u_j.
+template <class T, int KIND>
+__device__ __forceinline__ T product(int p, const T* v) {
+  if constexpr (KIND == ROW_MHD || KIND == ROW_VARS_MHD) {
+    switch (p) {
+      case 0: return v[3] * v[3] - v[0] * v[0];
+      case 1: return v[3] * v[4] - v[0] * v[1];
+      case 2: return v[3] * v[5] - v[0] * v[2];
+      case 3: return v[4] * v[4] - v[1] * v[1];
+      case 4: return v[4] * v[5] - v[1] * v[2];
+      case 5: return v[5] * v[5] - v[2] * v[2];
+      case 6: return v[0] * v[4] - v[3] * v[1];
+      case 7: return v[0] * v[5] - v[3] * v[2];
+      default: return v[1] * v[5] - v[4] * v[2];
+    }
+  } else {
+    switch (p) {
+      case 0: return v[0] * v[0];
+      case 1: return v[0] * v[1];
+      case 2: return v[0] * v[2];
+      case 3: return v[1] * v[1];
+      case 4: return v[1] * v[2];
+      default: return v[2] * v[2];
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
+
+// Forward real transform of the values z (one complex per pair of real samples,
+// in the pass-0 layout of Rfwd) and store the nkr half-spectrum entries.
+template <class T, int M, int E, int TX, class Rfwd, class Lds>
+__device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>* xrow,
+                                              const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twh,
+                                              cx<T>* __restrict__ dst, bool row_ok) {
+  constexpr int RfL = RLast<Rfwd>::value;
+  fft_regs<T, M, E, -1, Rfwd>(z, tr, ldsf, twM);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E / RfL; ++i)
+#pragma unroll
+    for (int r = 0; r < RfL; ++r) xrow[tr + i * TX + r * (M / RfL)] = z[i * RfL + r];
+  __syncthreads();
+  for (int k = tr; k <= M; k += TX) {
+    const cx<T> Zk = xrow[k == M ? 0 : k];
+    const cx<T> Zm = conj(xrow[k == 0 ? 0 : M - k]);
+    const cx<T> Ev = (Zk + Zm) * T(0.5);
+    const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
+    const cx<T> X = Ev + cmul(twh[k], Od);
+    if (row_ok) dst[k] = X;
+  }
+}
+
+template <class T, int NX, int KIND, int NF>
+__global__ void __launch_bounds__(128)
+xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
+            const cx<T>* __restrict__ twh) {
+  constexpr int M = NX / 2;
+  using RP = RowPlan<M>;
+  constexpr int E = RP::E;
+  constexpr int TX = M / E;
+  static_assert(TX <= 128, "row transform too long for one workgroup");
+  constexpr int RPW = 128 / TX;
+  using Rinv = typename RP::R;
+  using Rfwd = typename RReverse<Rinv>::type;
+  constexpr int R0 = RFirst<Rinv>::value;   // first inverse radix == last forward radix
+  constexpr int RL = RLast<Rinv>::value;    // last inverse radix == first forward radix
+  constexpr bool C2R_IN = (KIND == ROW_MHD || KIND == ROW_HD || KIND == ROW_C2R);
+  constexpr int NIN = RowKindInfo<KIND, NF>::NIN;
+  constexpr int NPROD = RowKindInfo<KIND, NF>::NPROD;
+
+  __shared__ cx<T> xb[RPW][M + 1];
+  __shared__ T red[2][128];
+
+  const int t = threadIdx.x;
+  const int rw = t / TX;
+  const int tr = t % TX;
+  const Geom& G = a.g;
+  const int64_t nb = (int64_t)gridDim.x;
+  const int64_t item = xcd_item(blockIdx.x, nb);
+  if (item >= a.blocks_per_run * a.nruns) return;
+  const int run = (int)(item / a.blocks_per_run);
+  const int64_t blk = item % a.blocks_per_run;
+  const int64_t row = blk * RPW + rw;
+  const bool row_ok = row < a.rows;
+  const RunBuf<T>& rb = runs[run];
+  T* const rbase = a.real_sel ? rb.rscr : rb.vars;
+  cx<T>* xrow = xb[rw];
+  auto ldsf = [&](int pos) -> cx<T>& { return xrow[pos]; };
+
+  // v[f][e]: the real samples 2j and 2j+1 of field f at the thread's positions
+  // j = tr + i*TX + r*(M/RL) (last inverse pass layout == first forward pass layout).
+  cx<T> v[NIN][E];
+
+  if constexpr (C2R_IN) {
+    const T scale = T(G.inv_n);
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      const cx<T>* __restrict__ src = rb.buf[a.in_sel] + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
+      __syncthreads();
+      for (int k = tr; k <= M; k += TX)
+        xrow[k] = (row_ok && k < a.in_ncols) ? src[k] : cx<T>{T(0), T(0)};
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E / R0; ++i) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const int p = tr + i * TX + r * (M / R0);
+          cx<T> Xk = xrow[p];
+          cx<T> Xm = xrow[M - p];
+          if (p == 0) { Xk.im = T(0); Xm.im = T(0); }  // c2r ignores Im of the DC and Nyquist bins
+          const cx<T> A = Xk + conj(Xm);
+          const cx<T> B = Xk - conj(Xm);
+          v[f][i * R0 + r] = A + mul_si<1>(cmulc(B, twh[p]));
+        }
+      }
+      fft_regs<T, M, E, 1, Rinv>(v[f], tr, ldsf, twM);
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[f][e] = v[f][e] * scale;
+    }
+    if (a.write_vars && row_ok) {
+#pragma unroll
+      for (int f = 0; f < NIN; ++f) {
+        T* __restrict__ vr = rbase + (int64_t)(a.real_out_f0 + f) * G.nr + row * NX;
+#pragma unroll
+        for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const int j = tr + i * TX + r * (M / RL);
+            *reinterpret_cast<cx<T>*>(vr + 2 * j) = v[f][i * RL + r];
+          }
+      }
+    }
+    if (a.want_max) {
+      T mu = T(0), mbm = T(0);
+      if (row_ok) {
+#pragma unroll
+        for (int f = 0; f < NIN; ++f)
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const T s1 = v[f][e].re * v[f][e].re, s2 = v[f][e].im * v[f][e].im;
+            if (f < 3) mu = nanmax(nanmax(mu, s1), s2);
+            else mbm = nanmax(nanmax(mbm, s1), s2);
+          }
+      }
+      red[0][t] = mu;
+      red[1][t] = mbm;
+      __syncthreads();
+      for (int s = 64; s > 0; s >>= 1) {
+        if (t < s) {
+          red[0][t] = nanmax(red[0][t], red[0][t + s]);
+          red[1][t] = nanmax(red[1][t], red[1][t + s]);
+        }
+        __syncthreads();
+      }
+      if (t == 0) {
+        rb.partial[2 * blk] = red[0][0];
+        rb.partial[2 * blk + 1] = red[1][0];
+      }
+    }
+  } else {
+    // real input rows (vars), loaded straight into the forward pass-0 layout
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      const T* __restrict__ vr = rbase + (int64_t)(a.in_f0 + f) * G.nr + row * NX;
+#pragma unroll
+      for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const int j = tr + i * TX + r * (M / RL);
+          v[f][i * RL + r] = row_ok ? *reinterpret_cast<const cx<T>*>(vr + 2 * j) : cx<T>{T(0), T(0)};
+        }
+    }
+  }
+
+  if constexpr (KIND == ROW_C2R) {
+    return;
+  } else {
+    // forward transforms: products (if any) then the inputs themselves (ROW_R2C / ROW_VARS_*)
+    constexpr int NOUT = NPROD + RowKindInfo<KIND, NF>::NSELF;
+#pragma unroll 1
+    for (int p = 0; p < NOUT; ++p) {
+      cx<T> z[E];
+      if (p < NPROD) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          T pr[NIN > 0 ? NIN : 1], pi[NIN > 0 ? NIN : 1];
+#pragma unroll
+          for (int f = 0; f < NIN; ++f) { pr[f] = v[f][e].re; pi[f] = v[f][e].im; }
+          z[e] = cx<T>{product<T, KIND>(p, pr), product<T, KIND>(p, pi)};
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < NIN; ++f)
+          if (f == p - NPROD)
+#pragma unroll
+            for (int e = 0; e < E; ++e) z[e] = v[f][e];
+      }
+      cx<T>* __restrict__ dst =
+          p < NPROD ? rb.buf[a.out_sel] + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
+                    : rb.buf[a.out2_sel] + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
+      row_r2c_store<T, M, E, TX, Rfwd>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spectral kernel: right-hand side at each mode + the Runge-Kutta stage update
+// ---------------------------------------------------------------------------
+enum SpecMode { SP_RK4 = 0, SP_LSRK54 = 1, SP_RHS = 2 };
+
+struct SpecArgs {
+  Geom g;
+  int nruns;
+  int mode;           // SpecMode
+  int stage;
+  int in_sel;         // stage input buffer (B_SOL / B_STIN / B_EXT_SOL)
+  int mask_input;     // 1: the stage input is dealiased (masked modes read as 0)
+  int zero_aliased;   // SP_RHS: write zeros into the aliased band of the input (dealias!)
+  int add_force;      // add the forcing table (MHD only, as addforcing! follows the advection)
+  int out_sel;        // SP_RHS: where N goes (B_EXT_N)
+  int out_f0;         // SP_RHS: first field of N written (component calls)
+  int f_lo, f_hi;     // SP_RHS: fields [f_lo, f_hi) written
+  int prod_sel;       // products buffer (B_WORK or B_SCR)
+  int self_sel;       // -1: diffusion on P_H(stage input); else R2C(vars) fields (B_SCR, at self_f0)
+  int self_f0;
+  double ca, cb;      // LSRK54 stage coefficients A_i, B_i (rounded to T in the kernel)
+  int64_t nitems;     // nruns * nc
+};
+
+template <class T>
+struct ModeK {
+  T kx, ky, kz, K, invK;
+};
+
+template <class T>
+__device__ __forceinline__ ModeK<T> mode_k(const Geom& G, int kx, int l, int m) {
+  ModeK<T> q;
+  q.kx = T((double)kx * G.multx);
+  q.ky = T((double)signed_index(l, G.ny) * G.multy);
+  q.kz = T((double)signed_index(m, G.nz) * G.multz);
+  // Krsq = kr^2 + l^2 + m^2 evaluated in T (utils.jl:73)
+  q.K = (q.kx * q.kx + q.ky * q.ky) + q.kz * q.kz;
+  q.invK = (kx == 0 && l == 0 && m == 0) ? T(0) : T(1) / q.K;
+  return q;
+}
+
+// N += -(K^p) * coef * s, promoted to double and rounded back (MHDSolver.jl:94,168)
+template <class T>
+__device__ __forceinline__ cx<T> add_diffusion(cx<T> n, T Kp, double coef, cx<T> s) {
+  const double c = -(double)Kp * coef;
+  return cx<T>{T((double)n.re + c * (double)s.re), T((double)n.im + c * (double)s.im)};
+}
+
+template <class T>
+__device__ __forceinline__ T ipow(T x, int n) {
+  T r = x;
+  for (int i = 1; i < n; ++i) r = r * x;
+  return r;
+}
+
+template <class T, int PHYS>
+__device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const SpecArgs& a,
+                                       int64_t q, const ModeK<T>& k, const cx<T>* sH, cx<T>* R) {
+  constexpr int NP = PHYS == PHYS_MHD ? 9 : 6;
+  const cx<T>* __restrict__ P = rb.buf[a.prod_sel];
+  cx<T> p[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) p[i] = P[(int64_t)i * G.nc + q];
+  const cx<T> zero{T(0), T(0)};
+  if constexpr (PHYS == PHYS_MHD) {
+    // D_j = sum_i i k_i T_ij ; N_u = D - k (k.D)/k^2  (UᵢUpdate!, MHDSolver.jl:63-83)
+    const cx<T> d0 = mul_si<1>(p[0] * k.kx + p[1] * k.ky + p[2] * k.kz);
+    const cx<T> d1 = mul_si<1>(p[1] * k.kx + p[3] * k.ky + p[4] * k.kz);
+    const cx<T> d2 = mul_si<1>(p[2] * k.kx + p[4] * k.ky + p[5] * k.kz);
+    const cx<T> kd = (d0 * k.kx + d1 * k.ky + d2 * k.kz) * k.invK;
+    R[0] = d0 - kd * k.kx;
+    R[1] = d1 - kd * k.ky;
+    R[2] = d2 - kd * k.kz;
+    // N_b_a = sum_{j != a} i k_j M_aj with M antisymmetric (BᵢUpdate!, MHDSolver.jl:147-158)
+    R[3] = mul_si<1>(p[6] * k.ky + p[7] * k.kz);
+    R[4] = mul_si<1>(p[8] * k.kz - p[6] * k.kx);
+    R[5] = mul_si<1>(zero - p[7] * k.kx - p[8] * k.ky);
+  } else {
+    // HD: N_u = -P(i k . (u u))  (HDSolver.jl:58-74)
+    const cx<T> d0 = mul_si<-1>(p[0] * k.kx + p[1] * k.ky + p[2] * k.kz);
+    const cx<T> d1 = mul_si<-1>(p[1] * k.kx + p[3] * k.ky + p[4] * k.kz);
+    const cx<T> d2 = mul_si<-1>(p[2] * k.kx + p[4] * k.ky + p[5] * k.kz);
+    const cx<T> kd = (d0 * k.kx + d1 * k.ky + d2 * k.kz) * k.invK;
+    R[0] = d0 - kd * k.kx;
+    R[1] = d1 - kd * k.ky;
+    R[2] = d2 - kd * k.kz;
+  }
+  constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const bool isb = f >= 3;
+    const double coef = isb ? rb.eta : rb.nu;
+    const int pw = isb ? rb.neta : rb.nnu;
+    R[f] = add_diffusion(R[f], k.K, coef, sH[f]);
+    if (pw > 1) R[f] = add_diffusion(R[f], ipow(k.K, pw), coef, sH[f]);
+  }
+}
+
+template <class T, int PHYS>
+__global__ void __launch_bounds__(256)
+spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
+  constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
+  const Geom& G = a.g;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= a.nitems) return;
+  const int run = (int)(gid / G.nc);
+  const int64_t idx = gid % G.nc;
+  const RunBuf<T>& rb = runs[run];
+  const int kx = (int)(idx % G.nkr);
+  const int64_t rest = idx / G.nkr;
+  const int l = (int)(rest % G.ny);
+  const int m = (int)(rest / G.ny);
+
+  // modes on the kx=0 and kx=nx/2 planes are handled in Hermitian pairs by the
+  // thread of the lower index, so the pair's stage inputs are read before either
+  // is overwritten.
+  const bool plane = (kx == 0 || kx == G.nx / 2);
+  int nmodes = 1;
+  int64_t qs[2] = {idx, idx};
+  int ls[2] = {l, l}, ms[2] = {m, m};
+  if (plane) {
+    const int lm = (G.ny - l) % G.ny, mm = (G.nz - m) % G.nz;
+    const int64_t mir = ((int64_t)mm * G.ny + lm) * G.nkr + kx;
+    if (mir < idx) return;
+    if (mir != idx) { nmodes = 2; qs[1] = mir; ls[1] = lm; ms[1] = mm; }
+  }
+
+  const cx<T>* __restrict__ in = rb.buf[a.in_sel];
+  const cx<T> zero{T(0), T(0)};
+  cx<T> s[2][NF];     // masked stage input at the (up to) two modes
+  bool kept[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    kept[j] = !a.mask_input || (kx < G.kx_keep && kept_y(G, ls[j]) && kept_z(G, ms[j]));
+    if (j < nmodes) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        s[j][f] = (kept[j] && a.self_sel < 0) ? in[(int64_t)f * G.nc + qs[j]] : zero;
+    }
+  }
+  if (a.zero_aliased) {
+    cx<T>* __restrict__ io = rb.buf[a.in_sel];
+    for (int j = 0; j < nmodes; ++j)
+      if (!kept[j])
+#pragma unroll
+        for (int f = 0; f < NF; ++f) io[(int64_t)f * G.nc + qs[j]] = zero;
+  }
+
+  for (int j = 0; j < nmodes; ++j) {
+    const int64_t q = qs[j];
+    const ModeK<T> k = mode_k<T>(G, kx, ls[j], ms[j]);
+    cx<T> sH[NF];
+    if (a.self_sel >= 0) {
+      const cx<T>* __restrict__ sf = rb.buf[a.self_sel];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sH[f] = sf[(int64_t)(a.self_f0 + f) * G.nc + q];
+    } else if (plane) {
+      // R2C(C2R(s)) = Hermitian projection on this plane
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sH[f] = (s[j][f] + conj(s[nmodes - 1 - j][f])) * T(0.5);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sH[f] = s[j][f];
+    }
+    cx<T> R[NF];
+    rhs_at<T, PHYS>(G, rb, a, q, k, sH, R);
+    if (PHYS == PHYS_MHD && a.add_force && rb.force) {
+      const int K = rb.force_K;
+      const int sl = signed_index(ls[j], G.ny), sm = signed_index(ms[j], G.nz);
+      if (kx <= K && sl >= -K && sl <= K && sm >= -K && sm <= K) {
+        const int64_t per = (int64_t)(K + 1) * (2 * K + 1) * (2 * K + 1);
+        const int64_t fi = ((int64_t)(sm + K) * (2 * K + 1) + (sl + K)) * (K + 1) + kx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[c] = R[c] + rb.force[c * per + fi];
+      }
+    }
+
+    if (a.mode == SP_RHS) {
+      cx<T>* __restrict__ out = rb.buf[a.out_sel];
+      for (int f = a.f_lo; f < a.f_hi; ++f) out[(int64_t)(a.out_f0 + f - a.f_lo) * G.nc + q] = R[f];
+      continue;
+    }
+    const T dt = T(rb.clk->dt);
+    cx<T>* __restrict__ sol = rb.buf[B_SOL];
+    cx<T>* __restrict__ acc = rb.buf[B_ACC];
+    cx<T>* __restrict__ stin = rb.buf[B_STIN];
+    if (a.mode == SP_RK4) {
+      // RK4substeps! / RK4update! (DyeModule.jl:62-91): acc holds the running sum
+      // RHS1/6 + RHS2/3 + RHS3/3 in the reference's left-to-right order.
+      const T half = dt / T(2);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int64_t o = (int64_t)f * G.nc + q;
+        const cx<T> base = kept[j] ? (a.stage == 0 ? s[j][f] : sol[o]) : zero;
+        if (a.stage == 0) {
+          acc[o] = cx<T>{R[f].re / T(6), R[f].im / T(6)};
+          if (kept[j]) stin[o] = base + R[f] * half;
+        } else if (a.stage == 1 || a.stage == 2) {
+          const cx<T> ac = acc[o];
+          acc[o] = cx<T>{ac.re + R[f].re / T(3), ac.im + R[f].im / T(3)};
+          if (kept[j]) stin[o] = base + R[f] * (a.stage == 1 ? half : dt);
+        } else {
+          const cx<T> ac = acc[o];
+          const cx<T> tot{ac.re + R[f].re / T(6), ac.im + R[f].im / T(6)};
+          sol[o] = base + tot * dt;
+        }
+      }
+    } else {
+      // LSRK54: S = A_i S + dt RHS ; sol += B_i S (on the dealiased sol)
+      const T A = T(a.ca), B = T(a.cb);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int64_t o = (int64_t)f * G.nc + q;
+        const cx<T> Sold = a.stage == 0 ? zero : acc[o];
+        const cx<T> Snew = Sold * A + R[f] * dt;
+        acc[o] = Snew;
+        sol[o] = s[j][f] + Snew * B;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels: CFL time step, maxima reduction + clock advance, mask
+// ---------------------------------------------------------------------------
+// getCFL! (integrator.jl:158-198): vmax = sqrt(max u^2) in T, dt = min(Coef*dl/vmax, t_diff)
+template <class T>
+__global__ void cfl_kernel(const RunBuf<T>* __restrict__ runs, int nruns, int phys) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  RunClock* c = runs[r].clk;
+  if (!c->use_cfl) return;
+  T v = sqrt(T(c->maxu2));
+  if (phys == PHYS_MHD) {
+    const T va = sqrt(T(c->maxb2));
+    v = (va > v || va != va) ? va : v;
+  }
+  const double cfl = c->cfl_coef * c->dl / (double)v;
+  const double dt = (cfl != cfl) ? cfl : (cfl < c->t_diff ? cfl : c->t_diff);
+  c->dt = (double)T(dt);
+}
+
+// reduce per-block maxima into the clock; optionally advance t and step
+template <class T>
+__global__ void __launch_bounds__(256)
+reduce_max_kernel(const RunBuf<T>* __restrict__ runs, int64_t nblocks, int advance) {
+  __shared__ T red[2][256];
+  const RunBuf<T>& rb = runs[blockIdx.x];
+  const int t = threadIdx.x;
+  T mu = T(0), mb = T(0);
+  for (int64_t i = t; i < nblocks; i += 256) {
+    mu = nanmax(mu, rb.partial[2 * i]);
+    mb = nanmax(mb, rb.partial[2 * i + 1]);
+  }
+  red[0][t] = mu;
+  red[1][t] = mb;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = nanmax(red[0][t], red[0][t + s]);
+      red[1][t] = nanmax(red[1][t], red[1][t + s]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    RunClock* c = rb.clk;
+    c->maxu2 = (double)red[0][0];
+    c->maxb2 = (double)red[1][0];
+    c->nan_flag = (red[0][0] != red[0][0] || red[1][0] != red[1][0]) ? 1 : 0;
+    if (advance) {
+      c->t = (double)(T(c->t) + T(c->dt));
+      c->step += 1;
+    }
+  }
+}
+
+// per-block max of u^2 and b^2 straight over vars (after the initial condition is set)
+template <class T>
+__global__ void __launch_bounds__(256)
+vars_max_kernel(const RunBuf<T>* __restrict__ runs, int64_t nr, int nf) {
+  __shared__ T red[2][256];
+  const RunBuf<T>& rb = runs[blockIdx.y];
+  const int t = threadIdx.x;
+  T mu = T(0), mb = T(0);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < nr; i += (int64_t)gridDim.x * 256) {
+    for (int f = 0; f < nf; ++f) {
+      const T v = rb.vars[(int64_t)f * nr + i];
+      if (f < 3) mu = nanmax(mu, v * v);
+      else mb = nanmax(mb, v * v);
+    }
+  }
+  red[0][t] = mu;
+  red[1][t] = mb;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = nanmax(red[0][t], red[0][t + s]);
+      red[1][t] = nanmax(red[1][t], red[1][t + s]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    rb.partial[2 * blockIdx.x] = red[0][0];
+    rb.partial[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+template <class T>
+__global__ void advance_clock_kernel(const RunBuf<T>* __restrict__ runs, int nruns) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  RunClock* c = runs[r].clk;
+  c->t = (double)(T(c->t) + T(c->dt));
+  c->step += 1;
+}
+
+// FP64 sums over the real fields: [0] sum |u|^2, [1] sum |b|^2, [2] sum u.b,
+// [3] max |f0|, per block; final pass sums in a fixed order (deterministic).
+template <class T>
+__global__ void __launch_bounds__(256)
+field_sums_kernel(const T* __restrict__ vars, int64_t nr, int nf, double* __restrict__ part) {
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+  double su = 0, sb = 0, sx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < nr; i += (int64_t)gridDim.x * 256) {
+    const double u0 = vars[i], u1 = vars[nr + i], u2 = vars[2 * nr + i];
+    su += u0 * u0 + u1 * u1 + u2 * u2;
+    if (nf == 6) {
+      const double b0 = vars[3 * nr + i], b1 = vars[4 * nr + i], b2 = vars[5 * nr + i];
+      sb += b0 * b0 + b1 * b1 + b2 * b2;
+      sx += u0 * b0 + u1 * b1 + u2 * b2;
+    }
+  }
+  red[0][t] = su; red[1][t] = sb; red[2][t] = sx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] += red[0][t + s];
+      red[1][t] += red[1][t + s];
+      red[2][t] += red[2][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[3 * blockIdx.x] = red[0][0];
+    part[3 * blockIdx.x + 1] = red[1][0];
+    part[3 * blockIdx.x + 2] = red[2][0];
+  }
+}
+
+// FP64 sum of products of pairs of real fields a_c * b_c (c < 3) and max |a0|:
+// used for helicities (u.curl u, A.b) and max |div b|.
+template <class T>
+__global__ void __launch_bounds__(256)
+dot_max_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t nr, int ncomp, double* __restrict__ part) {
+  __shared__ double red[2][256];
+  const int t = threadIdx.x;
+  double s = 0, mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < nr; i += (int64_t)gridDim.x * 256) {
+    if (b) {
+      for (int c = 0; c < ncomp; ++c) s += (double)a[c * nr + i] * (double)b[c * nr + i];
+    } else {
+      const double v = fabs((double)a[i]);
+      mx = (v > mx || v != v) ? v : mx;
+    }
+  }
+  red[0][t] = s; red[1][t] = mx;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      red[0][t] += red[0][t + st];
+      const double o = red[1][t + st];
+      red[1][t] = (o > red[1][t] || o != o) ? o : red[1][t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * blockIdx.x] = red[0][0];
+    part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+// Spectral operators for the diagnostics on a 3-component half spectrum X
+// (fields 0..2 of src) written to dst fields 0..2 (curl / vector potential) or
+// field 0 (divergence).  Wavenumbers in FP64.
+enum DiagOp { OP_CURL = 0, OP_VECPOT = 1, OP_DIV = 2 };
+template <class T>
+__global__ void __launch_bounds__(256)
+diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int op) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= G.nc) return;
+  const int kx = (int)(idx % G.nkr);
+  const int64_t rest = idx / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  const double k0 = kx * G.multx, k1 = signed_index(l, G.ny) * G.multy, k2 = signed_index(m, G.nz) * G.multz;
+  const cx<T> a0 = src[idx], a1 = src[G.nc + idx], a2 = src[2 * G.nc + idx];
+  auto ik = [](double kk, cx<T> z) { return cx<T>{T(-kk * (double)z.im), T(kk * (double)z.re)}; };
+  if (op == OP_DIV) {
+    const cx<T> d = ik(k0, a0) + ik(k1, a1) + ik(k2, a2);
+    dst[idx] = d;
+    return;
+  }
+  double s = 1.0;
+  if (op == OP_VECPOT) {
+    const double K = k0 * k0 + k1 * k1 + k2 * k2;
+    s = K == 0 ? 0.0 : 1.0 / K;
+  }
+  const cx<T> c0 = (ik(k1, a2) - ik(k2, a1)) * T(s);
+  const cx<T> c1 = (ik(k2, a0) - ik(k0, a2)) * T(s);
+  const cx<T> c2 = (ik(k0, a1) - ik(k1, a0)) * T(s);
+  dst[idx] = c0; dst[G.nc + idx] = c1; dst[2 * G.nc + idx] = c2;
+}
+
+// zero the aliased band of nf fields in place (FourierFlows dealias!)
+template <class T>
+__global__ void __launch_bounds__(256)
+dealias_kernel(Geom G, cx<T>* __restrict__ x, int nf) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= G.nc) return;
+  const int kx = (int)(idx % G.nkr);
+  const int64_t rest = idx / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  if (kx < G.kx_keep && kept_y(G, l) && kept_z(G, m)) return;
+  for (int f = 0; f < nf; ++f) x[(int64_t)f * G.nc + idx] = cx<T>{T(0), T(0)};
+}
+
+}  // namespace mhd

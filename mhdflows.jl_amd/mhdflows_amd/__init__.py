@@ -1,0 +1,29 @@
+"""MI355X-native hot path of MHDFlows.jl: 3D periodic incompressible HD/MHD
+right-hand side and RK4/LSRK54 time stepping, driven through the C ABI of
+libmhdflows_amd.so (include/mhdflows_amd.h)."""
+
+from ._lib import MHDError, lib  # noqa: F401
+from .problem import (  # noqa: F401
+    BiUpdate,
+    Diagnostics,
+    HDcalcN_advection,
+    MHDcalcN_advection,
+    MHDFlowsProblem,
+    ProbDiagnostic,
+    Problem,
+    SetUpProblemIC,
+    TimeIntegrator,
+    UiUpdate,
+    calcN,
+    getCFL,
+    nothingfunction,
+    refresh_vars,
+    set_forcing,
+    stepforward,
+)
+
+__all__ = [
+    "Problem", "SetUpProblemIC", "stepforward", "TimeIntegrator", "getCFL", "calcN",
+    "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
+    "Diagnostics", "set_forcing", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
+]

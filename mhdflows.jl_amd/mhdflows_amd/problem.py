@@ -1,0 +1,361 @@
+"""Host-side mirror of the MHDFlows.jl problem API on the MI355X device.
+
+Julia names ending in ``!`` drop the ``!`` here (``SetUpProblemIC!`` ->
+``SetUpProblemIC``, ``TimeIntegrator!`` -> ``TimeIntegrator`` ...).  Keyword
+arguments keep the reference's names, Greek letters included (``ν``, ``η``,
+``nν``, ``nη``); ASCII spellings (``nu``, ``eta``, ``nnu``, ``neta``) are
+accepted too.  Errors raise :class:`MHDError` (the reference's ``error(...)``).
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import MHDError, check, lib
+
+
+def nothingfunction(*args):
+    """Default ``calcF``/``usr_func`` hook (``src/pgen.jl:7``)."""
+    return None
+
+
+class _Clock:
+    """``prob.clock``: t, dt (stored in T on the device) and step."""
+
+    def __init__(self, prob):
+        self._p = prob
+
+    def _read(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().mhd_get_clock(self._p._h, C.byref(t), C.byref(dt), C.byref(st)))
+        return t.value, dt.value, st.value
+
+    @property
+    def t(self):
+        return self._read()[0]
+
+    @t.setter
+    def t(self, v):
+        _, dt, st = self._read()
+        check(lib().mhd_set_clock(self._p._h, float(v), dt, st))
+
+    @property
+    def dt(self):
+        return self._read()[1]
+
+    @dt.setter
+    def dt(self, v):
+        t, _, st = self._read()
+        check(lib().mhd_set_clock(self._p._h, t, float(v), st))
+
+    @property
+    def step(self):
+        return self._read()[2]
+
+    @step.setter
+    def step(self, v):
+        t, dt, _ = self._read()
+        check(lib().mhd_set_clock(self._p._h, t, dt, int(v)))
+
+
+class _Vars:
+    """``prob.vars``: real fields copied from the device on access."""
+
+    NAMES = ("ux", "uy", "uz", "bx", "by", "bz")
+
+    def __init__(self, prob):
+        self._p = prob
+
+    def field(self, f):
+        p = self._p
+        out = np.empty((p.grid.nz, p.grid.ny, p.grid.nx), dtype=p.T)
+        check(lib().mhd_get_vars(p._h, f, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def __getattr__(self, name):
+        if name in self.NAMES:
+            f = self.NAMES.index(name)
+            if f < self._p.nf:
+                return self.field(f)
+        raise AttributeError(name)
+
+
+class _Grid:
+    """Grid sizes and wavenumbers (``ThreeDGrid`` conventions, ``src/utils/utils.jl:42-98``)."""
+
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, T):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nkr = nx // 2 + 1
+        self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
+        self.T = T
+        self.dx = T(T(Lx) / T(nx))
+        self.dy = T(T(Ly) / T(ny))
+        self.dz = T(T(Lz) / T(nz))
+
+
+class _Params:
+    def __init__(self, nu, eta, nnu, neta, calcF):
+        self.ν = self.nu = nu
+        self.η = self.eta = eta
+        self.nν = self.nnu = nnu
+        self.nη = self.neta = neta
+        self.calcF = calcF
+        self.ux_ind, self.uy_ind, self.uz_ind = 1, 2, 3
+        self.bx_ind, self.by_ind, self.bz_ind = 4, 5, 6
+
+
+class _Flag:
+    def __init__(self, b):
+        self.b, self.e, self.vp, self.c, self.s = b, False, False, False, False
+
+
+class MHDFlowsProblem:
+    """A problem whose state lives in MI355X HBM, owned by the HIP library."""
+
+    def __init__(self, cfg, T, calcF):
+        self.cfg = cfg
+        self.T = T
+        self.C = np.complex64 if T == np.float32 else np.complex128
+        self.nf = 6 if cfg.physics == L.MHD_PHYS_MHD else 3
+        h = C.c_void_p()
+        check(lib().mhd_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.grid = _Grid(cfg.nx, cfg.ny, cfg.nz, cfg.Lx, cfg.Ly, cfg.Lz, T)
+        self.params = _Params(cfg.nu, cfg.eta, cfg.n_nu, 0, calcF)
+        self.flag = _Flag(cfg.physics == L.MHD_PHYS_MHD)
+        self.clock = _Clock(self)
+        self.vars = _Vars(self)
+        self.stepper = "RK4" if cfg.stepper == L.MHD_RK4 else "LSRK54"
+        self.usr_func = [nothingfunction]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().mhd_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- state ---------------------------------------------------------------
+    @property
+    def sol_shape(self):
+        g = self.grid
+        return (self.nf, g.nz, g.ny, g.nkr)
+
+    @property
+    def sol(self):
+        out = np.empty(self.sol_shape, dtype=self.C)
+        check(lib().mhd_get_sol(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    @sol.setter
+    def sol(self, value):
+        a = np.ascontiguousarray(value, dtype=self.C)
+        if a.shape != self.sol_shape:
+            raise MHDError(-1, f"sol must have shape {self.sol_shape}")
+        check(lib().mhd_set_sol(self._h, a.ctypes.data_as(C.c_void_p)))
+
+    def device_ptrs(self):
+        s, v = C.c_void_p(), C.c_void_p()
+        check(lib().mhd_device_ptrs(self._h, C.byref(s), C.byref(v)))
+        return s.value, v.value
+
+    def __repr__(self):
+        return (f"MHDFlows Problem on MI355X: {self.grid.nx}x{self.grid.ny}x{self.grid.nz}, "
+                f"B-field {'ON (Ideal MHD)' if self.flag.b else 'OFF'}, T={np.dtype(self.T).name}, "
+                f"timestepper {self.stepper}")
+
+
+def _pick(kw, names, default):
+    for n in names:
+        if n in kw:
+            return kw.pop(n)
+    return default
+
+
+def Problem(dev="MI355X", **kw):
+    """``Problem(dev; nx, ny, nz, Lx, Ly, Lz, ν, nν, η, nη, B_field, stepper, calcF, T, dt, ...)``
+    (``src/pgen.jl:64-127``).  Only plain HD/MHD on the MI355X device is supported here;
+    the flags of other physics raise as unsupported.
+
+    ``aliased_fraction`` is accepted and ignored, as the reference does not forward
+    it to the grid (``src/pgen.jl:91`` vs ``:107``): the 1/3 rule always applies.
+    ``nη`` is accepted and ignored, as the reference does not forward it to the
+    parameters (``src/pgen.jl:114-116``).
+    """
+    if dev not in ("MI355X", "GPU"):
+        raise MHDError(-1, f"unknown device {dev!r}: this package drives the MI355X device")
+    nx = int(_pick(kw, ["nx"], 64))
+    ny = int(_pick(kw, ["ny"], nx))
+    nz = int(_pick(kw, ["nz"], nx))
+    Lx = float(_pick(kw, ["Lx"], 2 * math.pi))
+    Ly = float(_pick(kw, ["Ly"], Lx))
+    Lz = float(_pick(kw, ["Lz"], Lx))
+    nu = float(_pick(kw, ["ν", "nu"], 0.0))
+    nnu = int(_pick(kw, ["nν", "nnu"], 0))
+    eta = float(_pick(kw, ["η", "eta"], 0.0))
+    _pick(kw, ["nη", "neta"], 0)
+    B = bool(_pick(kw, ["B_field"], False))
+    for flag in ("EMHD", "Compressibility", "Shear", "VP_method", "Dye_Module"):
+        if _pick(kw, [flag], False):
+            raise MHDError(-2, f"{flag} is not part of the MI355X hot path")
+    stepper = _pick(kw, ["stepper"], "RK4")
+    calcF = _pick(kw, ["calcF"], nothingfunction)
+    T = _pick(kw, ["T"], np.float32)
+    dt = float(_pick(kw, ["dt"], 0.0))
+    _pick(kw, ["aliased_fraction"], 1 / 3)
+    device = int(_pick(kw, ["device"], 0))
+    budget = int(_pick(kw, ["hbm_budget"], 0))
+    for k in ("usr_vars", "usr_params", "usr_func", "cₛ"):
+        kw.pop(k, None)
+    if kw:
+        raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
+    T = np.dtype(T).type
+    if T not in (np.float32, np.float64):
+        raise MHDError(-1, "T must be Float32 or Float64")
+    if stepper not in ("RK4", "LSRK54", "LSRK4"):
+        raise MHDError(-1, f"stepper {stepper!r} is not supported on the MI355X device")
+    cfg = L.Config(nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, nu=nu, eta=eta if B else 0.0,
+                   n_nu=nnu, n_eta=0, precision=L.MHD_F32 if T == np.float32 else L.MHD_F64,
+                   physics=L.MHD_PHYS_MHD if B else L.MHD_PHYS_HD,
+                   stepper=L.MHD_RK4 if stepper == "RK4" else L.MHD_LSRK54,
+                   device=device, stream=None, hbm_budget=budget, dt=dt)
+    return MHDFlowsProblem(cfg, T, calcF)
+
+
+def SetUpProblemIC(prob, ux=None, uy=None, uz=None, bx=None, by=None, bz=None, **kw):
+    """``SetUpProblemIC!`` (``src/utils/IC.jl:41-109``): real fields in ``[z][y][x]`` order."""
+    g = prob.grid
+    arrs = [ux, uy, uz] + ([bx, by, bz] if prob.nf == 6 else [])
+    keep = []
+    ptrs = (C.c_void_p * prob.nf)()
+    for f, a in enumerate(arrs):
+        if a is None or (isinstance(a, (list, tuple)) and len(a) == 0):
+            ptrs[f] = None
+            continue
+        a = np.ascontiguousarray(a, dtype=prob.T)
+        if a.shape != (g.nz, g.ny, g.nx):
+            raise MHDError(-1, f"field {f} must have shape {(g.nz, g.ny, g.nx)}")
+        keep.append(a)
+        ptrs[f] = a.ctypes.data
+    check(lib().mhd_set_ic_real(prob._h, ptrs))
+
+
+def stepforward(prob, nsteps=1):
+    """``stepforward!`` (``src/timestepper/timestepper.jl:4-6``) with the problem's stepper."""
+    check(lib().mhd_step(prob._h, int(nsteps)))
+
+
+def getCFL(prob, t_diff=None, Coef=0.3):
+    """``getCFL!`` (``src/integrator.jl:158-198``); sets and returns ``clock.dt``."""
+    dt = C.c_double()
+    check(lib().mhd_cfl_dt(prob._h, float(Coef), C.byref(dt)))
+    return dt.value
+
+
+def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothingfunction, diags=(),
+                   dynamic_dashboard=False, loop_number=100, save=False, save_loc="", filename="",
+                   file_number=0, dump_dt=0):
+    """``TimeIntegrator!`` (``src/integrator.jl:31-156``): at most ``N0+1`` steps while ``t <= t0``.
+
+    The hot loop runs on the device (dt from ``getCFL!`` computed there).  Diagnostics
+    callables in ``diags`` and ``prob.usr_func`` hooks are called once per step and force
+    a per-step loop on the host; without them the loop has no host synchronisation.
+    """
+    if save:
+        if len(save_loc) == 0 or len(filename) == 0 or dump_dt == 0:
+            raise MHDError(-1, "Save Function Turned ON but save_loc/filename/dump_dt is not declared!\n")
+        raise MHDError(-2, "HDF5 output is not available in this build")
+    if CFL_function is not nothingfunction:
+        if usr_dt > 0.0:
+            raise MHDError(-1, "User define both CFL_function and usr_dt")
+        raise MHDError(-2, "user CFL functions run on the host and are not supported on the device loop")
+    hooks = [f for f in prob.usr_func if f is not nothingfunction]
+    if not diags and not hooks:
+        steps = C.c_int64()
+        check(lib().mhd_integrate(prob._h, float(t0), int(N0), float(usr_dt), float(CFL_Coef), C.byref(steps)))
+        return steps.value
+    # per-step loop with host hooks
+    taken = 0
+    prob.clock.step = 0
+    if usr_dt != 0.0:
+        prob.clock.dt = usr_dt
+    while N0 >= prob.clock.step and t0 >= prob.clock.t:
+        if usr_dt == 0.0:
+            getCFL(prob, Coef=CFL_Coef)
+        stepforward(prob, 1)
+        taken += 1
+        for d in diags:
+            d(prob)
+        for f in hooks:
+            f(prob)
+    return taken
+
+
+def _dev_ptr(x):
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    raise MHDError(-1, "device arrays are passed as integer device pointers")
+
+
+def calcN(prob, N, sol, t=0.0):
+    """``MHDcalcN!``/``HDcalcN!`` (``src/pgen.jl:153-181``) on device pointers ``N``, ``sol``."""
+    check(lib().mhd_calcN(prob._h, _dev_ptr(N), _dev_ptr(sol), float(t)))
+
+
+def MHDcalcN_advection(prob, N, sol, t=0.0):
+    """``MHDcalcN_advection!`` (``src/Solver/MHDSolver.jl:330-351``)."""
+    check(lib().mhd_advection(prob._h, _dev_ptr(N), _dev_ptr(sol), float(t)))
+
+
+HDcalcN_advection = MHDcalcN_advection
+
+_DIRS = {"x": 0, "y": 1, "z": 2}
+
+
+def UiUpdate(prob, N, direction="x"):
+    """``UᵢUpdate!(N, sol, t, clock, vars, params, grid; direction)`` (``MHDSolver.jl:27-103``)."""
+    if direction not in _DIRS:
+        raise MHDError(-1, "Warning : Unknown direction is declerad")
+    check(lib().mhd_u_update(prob._h, _dev_ptr(N), _DIRS[direction]))
+
+
+def BiUpdate(prob, N, direction="x"):
+    """``BᵢUpdate!`` (``MHDSolver.jl:106-177``)."""
+    if direction not in _DIRS:
+        raise MHDError(-1, "Warning : Unknown direction is declerad")
+    check(lib().mhd_b_update(prob._h, _dev_ptr(N), _DIRS[direction]))
+
+
+def Diagnostics(prob):
+    """FP64 energies, helicities and divergences of the current ``vars``."""
+    d = L.Diag()
+    check(lib().mhd_diagnostics(prob._h, C.byref(d)))
+    return {k: getattr(d, k) for k in ("E_kin", "E_mag", "H_kin", "H_mag", "H_cross", "max_div_b", "max_div_u")} | {
+        "has_nan": bool(d.has_nan)}
+
+
+def ProbDiagnostic(prob):
+    """``ProbDiagnostic`` (``src/utils/UserInterface.jl:65-86``): KE (and ME), 3 significant digits."""
+    d = Diagnostics(prob)
+    if d["has_nan"] or math.isnan(d["E_kin"]):
+        raise MHDError(-5, "detected NaN! Quit the simulation right now.")
+
+    def r3(x):
+        return float(f"{x:.3g}")
+
+    return (r3(d["E_kin"]), r3(d["E_mag"])) if prob.flag.b else r3(d["E_kin"])
+
+
+def set_forcing(prob, seed, kf=2.0, amp=1.0):
+    """Seeded large-scale solenoidal forcing (the ``calcF!`` hook, ``src/pgen.jl:231-234``)."""
+    check(lib().mhd_set_forcing(prob._h, int(seed), float(kf), float(amp)))
+
+
+def refresh_vars(prob):
+    check(lib().mhd_refresh_vars(prob._h))

@@ -38,10 +38,11 @@ def t_diff(prob, coef):
     else:
         vi, nv = prob.nu, prob.nnu
     g = prob.grid
-    dl = float(min(g.dx, g.dy, g.dz))
+    dl = min(g.dx, g.dy, g.dz)          # type T
     if vi == 0:
         return math.inf
-    return coef * dl ** nv / vi if nv > 1 else coef * dl ** 2 / vi
+    p = dl ** nv if nv > 1 else dl * dl  # evaluated in T, then promoted (integrator.jl:72)
+    return coef * float(p) / vi
 
 
 def dV(grid):

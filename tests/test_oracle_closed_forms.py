@@ -181,8 +181,9 @@ def test_cfl_and_integrator_step_count():
     zero = np.zeros_like(X)
     p.set_ic(2.0 * np.sin(Y), zero, zero, zero, 0.5 * np.cos(Z), zero)
     td = t_diff(p, 0.3)
-    dl = float(np.float32(np.float32(2 * math.pi) / np.float32(8)))
-    assert math.isclose(td, 0.3 * dl * dl / 0.1, rel_tol=1e-12)
+    dl32 = np.float32(np.float32(2 * math.pi) / np.float32(8))
+    dl = float(dl32)
+    assert td == 0.3 * float(dl32 * dl32) / 0.1  # dl^2 in Float32, then promoted (integrator.jl:72)
     vmax = float(np.float32(math.sqrt(float(np.max(p.vars[0] ** 2)))))
     assert cfl_dt(p, td, 0.3) == np.float32(min(0.3 * dl / vmax, td))
     time_integrator(p, 1e9, 4, usr_dt=1e-3)
