@@ -107,6 +107,10 @@ int32_t mhd_set_ic_real(mhd_handle* h, const void* const* fields);
 /* Copy prob.sol (nf*nz*ny*nkr complex T) from / to host memory. */
 int32_t mhd_get_sol(mhd_handle* h, void* host_out);
 int32_t mhd_set_sol(mhd_handle* h, const void* host_in);
+/* Zero prob.sol and write nmodes sparse half-spectrum entries: idx holds (kx, l, m)
+ * array indices per entry, values holds nf complex T per entry (field-fastest).
+ * vars are refreshed from the dealiased state (a spectral SetUpProblemIC!). */
+int32_t mhd_set_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values);
 /* Copy prob.vars real field f (0..nf-1) to host memory. */
 int32_t mhd_get_vars(mhd_handle* h, int32_t field, void* host_out);
 /* Device pointers of the handle's sol (nf fields) and vars (nf real fields). */
@@ -174,6 +178,7 @@ int32_t mhd_batch_create(const mhd_config* cfgs, int32_t nruns, mhd_batch** out)
 int32_t mhd_batch_destroy(mhd_batch* b);
 int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields);
 int32_t mhd_batch_set_sol(mhd_batch* b, int32_t run, const void* host_in);
+int32_t mhd_batch_set_sol_modes(mhd_batch* b, int32_t run, int64_t nmodes, const int32_t* idx, const void* values);
 int32_t mhd_batch_get_sol(mhd_batch* b, int32_t run, void* host_out);
 int32_t mhd_batch_get_clock(mhd_batch* b, int32_t run, double* t, double* dt, int64_t* step);
 /* Set per-run CFL control (coef > 0: dt from getCFL! before every step). */

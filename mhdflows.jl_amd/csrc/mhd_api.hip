@@ -63,6 +63,12 @@ int check_handle(mhd_handle* h) {
 
 }  // namespace
 
+#define BATCH_RUN(b, run)                                                          \
+  if (!(b)) return fail(E_INVALID, "batch is NULL");                              \
+  if ((run) < 0 || (run) >= (int)(b)->where.size()) return fail(E_INVALID, "run out of range"); \
+  auto& G_ = (b)->groups[(b)->where[(run)].first];                                 \
+  const int L_ = (b)->where[(run)].second;
+
 extern "C" {
 
 int32_t mhd_abi_version(void) { return MHD_ABI_VERSION; }
@@ -121,6 +127,16 @@ int32_t mhd_set_sol(mhd_handle* h, const void* host_in) {
   MHD_TRY(check_handle(h));
   if (!host_in) return fail(E_INVALID, "input pointer is NULL");
   return h->e->set_sol(0, host_in);
+}
+int32_t mhd_set_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values) {
+  MHD_TRY(check_handle(h));
+  if (nmodes > 0 && (!idx || !values)) return fail(E_INVALID, "idx and values are required");
+  return h->e->set_sol_modes(0, nmodes, idx, values);
+}
+int32_t mhd_batch_set_sol_modes(mhd_batch* b, int32_t run, int64_t nmodes, const int32_t* idx, const void* values) {
+  BATCH_RUN(b, run);
+  if (nmodes > 0 && (!idx || !values)) return fail(E_INVALID, "idx and values are required");
+  return G_.e->set_sol_modes(L_, nmodes, idx, values);
 }
 int32_t mhd_get_vars(mhd_handle* h, int32_t field, void* host_out) {
   MHD_TRY(check_handle(h));
@@ -293,11 +309,6 @@ int32_t mhd_batch_destroy(mhd_batch* b) {
   return 0;
 }
 
-#define BATCH_RUN(b, run)                                                          \
-  if (!(b)) return fail(E_INVALID, "batch is NULL");                              \
-  if ((run) < 0 || (run) >= (int)(b)->where.size()) return fail(E_INVALID, "run out of range"); \
-  auto& G_ = (b)->groups[(b)->where[(run)].first];                                 \
-  const int L_ = (b)->where[(run)].second;
 
 int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields) {
   BATCH_RUN(b, run);

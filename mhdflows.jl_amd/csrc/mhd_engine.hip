@@ -75,7 +75,7 @@ int64_t run_bytes(const Geom& g, int phys, int prec) {
   const int s = prec == 0 ? 4 : 8;
   const int nf = phys == PHYS_MHD ? 6 : 3;
   const int np = phys == PHYS_MHD ? 9 : 6;
-  const int64_t rb = (int64_t)row_rows_per_block(g.nx);
+  const int64_t rb = prec == 0 ? (int64_t)row_rows_per_block<float>(g.nx) : (int64_t)row_rows_per_block<double>(g.nx);
   const int64_t blocks = ((int64_t)g.ny * g.nz + rb - 1) / rb;
   return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s + 4096;
 }
@@ -151,7 +151,7 @@ class Engine : public EngineBase {
     twM = twiddles<T>(device, g.nx / 2, 0, &err);
     twh = twiddles<T>(device, g.nx, 1, &err);
     if (err) return err;
-    const int64_t rpb = row_rows_per_block(g.nx);
+    const int64_t rpb = row_rows_per_block<T>(g.nx);
     row_blocks = ((int64_t)g.ny * g.nz + rpb - 1) / rpb;
     rbh.resize(nr_);
     force_tabs.assign(nr_, nullptr);
@@ -457,6 +457,33 @@ class Engine : public EngineBase {
     MHD_HIP(hipStreamSynchronize(st));
     return 0;
   }
+  // zero sol, scatter sparse modes, then refresh vars (and the CFL maxima) from it
+  int set_sol_modes(int run, int64_t nmodes, const int32_t* idx, const void* vals) override {
+    MHD_TRY(check_run(run));
+    if (nmodes < 0) return fail(E_INVALID, "nmodes must be >= 0");
+    for (int64_t i = 0; i < nmodes; ++i) {
+      if (idx[3 * i] < 0 || idx[3 * i] >= g.nkr || idx[3 * i + 1] < 0 || idx[3 * i + 1] >= g.ny ||
+          idx[3 * i + 2] < 0 || idx[3 * i + 2] >= g.nz)
+        return fail(E_INVALID, "mode index outside the half spectrum");
+    }
+    RunBuf<T>& b = rbh[run];
+    MHD_HIP(hipMemsetAsync(b.buf[B_SOL], 0, sizeof(cx<T>) * g.nc * nf, st));
+    if (nmodes > 0) {
+      void *di = nullptr, *dv = nullptr;
+      MHD_HIP(hipMalloc(&di, sizeof(int32_t) * 3 * nmodes));
+      MHD_HIP(hipMalloc(&dv, sizeof(cx<T>) * nf * nmodes));
+      MHD_HIP(hipMemcpyAsync(di, idx, sizeof(int32_t) * 3 * nmodes, hipMemcpyHostToDevice, st));
+      MHD_HIP(hipMemcpyAsync(dv, vals, sizeof(cx<T>) * nf * nmodes, hipMemcpyHostToDevice, st));
+      MHD_TRY(timed("scatter", 0, [&] {
+        return launch_scatter<T>(g, b.buf[B_SOL], nf, nmodes, (const int32_t*)di, (const cx<T>*)dv, st);
+      }));
+      MHD_HIP(hipStreamSynchronize(st));
+      hipFree(di);
+      hipFree(dv);
+    }
+    return refresh_vars_run(run);
+  }
+
   int get_vars(int run, int f, void* host) override {
     MHD_TRY(check_run(run));
     if (f < 0 || f >= nf) return fail(E_INVALID, "field index out of range");
@@ -640,6 +667,8 @@ class Engine : public EngineBase {
   }
 
   // ---- vars refresh and diagnostics -------------------------------------------
+  int refresh_vars_run(int /*run*/) { return refresh_vars(); }
+
   int refresh_vars() override {
     MHD_TRY(colpass("zpass_inv", 2, 1, B_SOL, B_WORK, 0, 0, nf, true));
     MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, true));

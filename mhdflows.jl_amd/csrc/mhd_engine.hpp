@@ -50,6 +50,7 @@ class EngineBase {
   virtual int set_ic_real(int run, const void* const* fields) = 0;
   virtual int get_sol(int run, void* host) = 0;
   virtual int set_sol(int run, const void* host) = 0;
+  virtual int set_sol_modes(int run, int64_t nmodes, const int32_t* idx, const void* vals) = 0;
   virtual int get_vars(int run, int f, void* host) = 0;
   virtual int device_ptrs(int run, void** sol, void** vars) = 0;
   virtual int get_clock(int run, double* t, double* dt, int64_t* step) = 0;

@@ -78,6 +78,25 @@ template <class T, int S> struct Bfly<T, 4, S> {
   }
 };
 
+template <class T, int S> struct Bfly<T, 6, S> {
+  static __device__ __forceinline__ void run(cx<T>* x) {
+    // 6 = 3 x 2: A_j2[k1] = DFT3_j1(x[2 j1 + j2]); A_1[k1] *= w6^k1; y[k1 + 3 k2] = A_0 +- A_1
+    const T c = T(0.86602540378443864676372317075293618);  // sqrt(3)/2
+    cx<T> a[3] = {x[0], x[2], x[4]};
+    cx<T> b[3] = {x[1], x[3], x[5]};
+    Bfly<T, 3, S>::run(a);
+    Bfly<T, 3, S>::run(b);
+    const cx<T> w1 = {T(0.5), T(S) * c}, w2 = {T(-0.5), T(S) * c};
+    b[1] = cmul(b[1], w1);
+    b[2] = cmul(b[2], w2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x[k] = a[k] + b[k];
+      x[k + 3] = a[k] - b[k];
+    }
+  }
+};
+
 template <class T, int S> struct Bfly<T, 8, S> {
   static __device__ __forceinline__ void run(cx<T>* x) {
     const T h = T(0.70710678118654752440084436210484904);  // sqrt(2)/2
@@ -177,19 +196,36 @@ template <int M> struct RowPlan;
   };
 MHD_ROWPLAN(4, 4, 4)
 MHD_ROWPLAN(8, 8, 8)
-MHD_ROWPLAN(12, 12, 4, 3)
+MHD_ROWPLAN(12, 6, 6, 2)
 MHD_ROWPLAN(16, 8, 8, 2)
-MHD_ROWPLAN(24, 12, 4, 3, 2)
+MHD_ROWPLAN(24, 6, 6, 2, 2)
 MHD_ROWPLAN(32, 8, 8, 4)
-MHD_ROWPLAN(48, 12, 4, 4, 3)
+MHD_ROWPLAN(48, 6, 6, 2, 2, 2)
 MHD_ROWPLAN(64, 8, 8, 8)
-MHD_ROWPLAN(96, 12, 4, 4, 2, 3)
+MHD_ROWPLAN(96, 6, 6, 2, 2, 2, 2)
 MHD_ROWPLAN(128, 8, 8, 8, 2)
-MHD_ROWPLAN(192, 12, 4, 4, 4, 3)
+MHD_ROWPLAN(192, 6, 6, 2, 2, 2, 2, 2)
 MHD_ROWPLAN(256, 8, 8, 8, 4)
-MHD_ROWPLAN(384, 12, 4, 4, 4, 2, 3)
+MHD_ROWPLAN(384, 6, 6, 2, 2, 2, 2, 2, 2)
 MHD_ROWPLAN(512, 8, 8, 8, 8)
 #undef MHD_ROWPLAN
+
+// Float64 rows keep four (powers of two) or six complex values per field per
+// thread, so six resident fields stay inside the architectural VGPRs.
+template <class T, int M> struct RowPlanT : RowPlan<M> {};
+#define MHD_ROWPLAN_D(M_, E_, ...)                   \
+  template <> struct RowPlanT<double, M_> {          \
+    static constexpr int E = E_;                     \
+    using R = Radices<__VA_ARGS__>;                  \
+  };
+MHD_ROWPLAN_D(8, 4, 4, 2)
+MHD_ROWPLAN_D(16, 4, 4, 4)
+MHD_ROWPLAN_D(32, 4, 4, 4, 2)
+MHD_ROWPLAN_D(64, 4, 4, 4, 4)
+MHD_ROWPLAN_D(128, 4, 4, 4, 4, 2)
+MHD_ROWPLAN_D(256, 4, 4, 4, 4, 4)
+MHD_ROWPLAN_D(512, 4, 4, 4, 4, 4, 2)
+#undef MHD_ROWPLAN_D
 
 template <class L> struct RFirst;
 template <int R0, int... Rs> struct RFirst<Radices<R0, Rs...>> { static constexpr int value = R0; };

@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(128)
 xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
             const cx<T>* __restrict__ twh) {
   constexpr int M = NX / 2;
-  using RP = RowPlan<M>;
+  using RP = RowPlanT<T, M>;
   constexpr int E = RP::E;
   constexpr int TX = M / E;
   static_assert(TX <= 128, "row transform too long for one workgroup");
@@ -759,6 +759,19 @@ diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ 
   const cx<T> c1 = (ik(k2, a0) - ik(k0, a2)) * T(s);
   const cx<T> c2 = (ik(k0, a1) - ik(k1, a0)) * T(s);
   dst[idx] = c0; dst[G.nc + idx] = c1; dst[2 * G.nc + idx] = c2;
+}
+
+// scatter sparse half-spectrum entries (kx, l, m) of nf fields into x (already zeroed)
+template <class T>
+__global__ void __launch_bounds__(256)
+scatter_modes_kernel(Geom G, cx<T>* __restrict__ x, int nf, int64_t nmodes, const int32_t* __restrict__ idx,
+                     const cx<T>* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nmodes) return;
+  const int kx = idx[3 * i], l = idx[3 * i + 1], m = idx[3 * i + 2];
+  if (kx < 0 || kx >= G.nkr || l < 0 || l >= G.ny || m < 0 || m >= G.nz) return;
+  const int64_t q = ((int64_t)m * G.ny + l) * G.nkr + kx;
+  for (int f = 0; f < nf; ++f) x[(int64_t)f * G.nc + q] = vals[i * nf + f];
 }
 
 // zero the aliased band of nf fields in place (FourierFlows dealias!)

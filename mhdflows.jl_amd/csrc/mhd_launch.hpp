@@ -24,14 +24,15 @@ template <class T> inline int col_threads(int n) {
   }
   return strip_width<T>() * (n / e);
 }
-inline int row_rows_per_block(int nx) {
+template <class T> inline int row_rows_per_block(int nx) {
+  // must match RowPlanT<T, nx/2>::E (mhd_fft.hpp)
   const int m = nx / 2;
   int e = 8;
-  switch (m) {
-    case 4: e = 4; break;
-    case 12: case 24: case 48: case 96: case 192: case 384: e = 12; break;
-    default: e = 8;
-  }
+  const bool three = (m % 3 == 0);
+  if (m == 4) e = 4;
+  else if (three) e = 6;
+  else if (sizeof(T) == 8) e = 4;
+  else e = 8;
   const int tx = m / e;
   return 128 / tx;
 }
@@ -55,6 +56,9 @@ template <class T>
 hipError_t launch_diag_spectral(const Geom& g, const cx<T>* src, cx<T>* dst, int op, hipStream_t s);
 template <class T>
 hipError_t launch_dealias(const Geom& g, cx<T>* x, int nf, hipStream_t s);
+template <class T>
+hipError_t launch_scatter(const Geom& g, cx<T>* x, int nf, int64_t nmodes, const int32_t* idx, const cx<T>* vals,
+                          hipStream_t s);
 
 enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3 };
 

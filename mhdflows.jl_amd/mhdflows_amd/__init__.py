@@ -12,6 +12,7 @@ from .problem import (  # noqa: F401
     ProbDiagnostic,
     Problem,
     SetUpProblemIC,
+    SetUpProblemIC_modes,
     TimeIntegrator,
     UiUpdate,
     calcN,
@@ -23,7 +24,7 @@ from .problem import (  # noqa: F401
 )
 
 __all__ = [
-    "Problem", "SetUpProblemIC", "stepforward", "TimeIntegrator", "getCFL", "calcN",
+    "Problem", "SetUpProblemIC", "SetUpProblemIC_modes", "stepforward", "TimeIntegrator", "getCFL", "calcN",
     "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
     "Diagnostics", "set_forcing", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
 ]

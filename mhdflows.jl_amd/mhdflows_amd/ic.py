@@ -96,6 +96,26 @@ class ModeSet:
               * np.exp(1j * sh * kr)[None, None, :])
         return ph[None]
 
+    def half_entries(self, n, nf=6):
+        """Sparse half-spectrum entries of the fields sampled on an ``n^3`` grid:
+        ``(idx, vals)`` with ``idx`` (M, 3) int32 array indices (kx, l, m) and ``vals``
+        (M, nf) complex128 (R2C convention, origin ``-L/2``)."""
+        N = float(n) ** 3
+        amps = [self.au] + ([self.ab] if nf == 6 else [])
+        idx, vals = [], []
+        for sgn in (1, -1):
+            k = sgn * self.kvec
+            sel = k[:, 0] >= 0
+            if sgn == -1:
+                sel &= self.kvec[:, 0] == 0  # kx = 0 plane needs both members of the pair
+            ks = k[sel]
+            ph = np.exp(-1j * math.pi * ks.sum(1)) if abs(self.L - 2 * math.pi) < 1e-15 else \
+                np.exp(-1j * (self.L / 2) * (2 * math.pi / self.L) * ks.sum(1))
+            v = np.stack([(A[sel, c] if sgn == 1 else np.conj(A[sel, c])) for A in amps for c in range(3)], 1)
+            idx.append(np.stack([ks[:, 0], ks[:, 1] % n, ks[:, 2] % n], 1))
+            vals.append(N * v * ph[:, None])
+        return np.concatenate(idx).astype(np.int32), np.concatenate(vals)
+
     def spectrum_on_grid(self, n, nf=6):
         """Half spectrum of the real fields sampled on ``x_i = -L/2 + i L/n`` (R2C convention)."""
         return self.half_spectrum(n, nf) * self._origin_phase(n)

@@ -244,6 +244,16 @@ def SetUpProblemIC(prob, ux=None, uy=None, uz=None, bx=None, by=None, bz=None, *
     check(lib().mhd_set_ic_real(prob._h, ptrs))
 
 
+def SetUpProblemIC_modes(prob, idx, vals):
+    """Spectral initial condition from sparse half-spectrum entries (see ``ic.ModeSet.half_entries``)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=prob.C)
+    if idx.ndim != 2 or idx.shape[1] != 3 or vals.shape != (idx.shape[0], prob.nf):
+        raise MHDError(-1, "idx must be (M,3) and vals (M,nf)")
+    check(lib().mhd_set_sol_modes(prob._h, idx.shape[0], idx.ctypes.data_as(C.c_void_p),
+                                  vals.ctypes.data_as(C.c_void_p)))
+
+
 def stepforward(prob, nsteps=1):
     """``stepforward!`` (``src/timestepper/timestepper.jl:4-6``) with the problem's stepper."""
     check(lib().mhd_step(prob._h, int(nsteps)))
