@@ -96,6 +96,11 @@ def main():
     import ctypes as C
     step_bytes = M.lib().mhd_step_bytes_model(C.byref(cfg))
 
+    log("bench: autotuning launch variants")
+    tune_report = M.autotune(p, 5)
+    log(tune_report.strip())
+    chosen = M.variants(p)
+
     log(f"bench: warmup {args.warmup} steps")
     M.TimeIntegrator(p, math.inf, max(args.warmup, 1) - 1, CFL_Coef=0.3)
     hip().hipDeviceSynchronize()
@@ -151,6 +156,8 @@ def main():
         "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "variants": {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
+                     "zpass_fwd": chosen[3], "xrow_fused": chosen[4]},
         "cpu_baseline": cpu,
         "zone_updates_per_s": round(value * n ** 3, 1),
         "E_kin": d["E_kin"], "E_mag": d["E_mag"],

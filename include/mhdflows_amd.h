@@ -161,6 +161,16 @@ int32_t mhd_refresh_vars(mhd_handle* h);
 int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp);
 
 /* ---- measurement -------------------------------------------------------------- */
+/* Autotuner: time every compiled variant of the hot launches (column-pass strip
+ * widths, row-transform plans) with reps launches each, keep the fastest for this
+ * grid and precision (for the process) and rebuild the captured step.  A text
+ * report of the timings is written to report (may be NULL). */
+int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len);
+/* Set (v5 != NULL; negative entries keep the current value) and/or read (out5 != NULL)
+ * the launch variants {zpass_inv, ypass_inv, ypass_fwd, zpass_fwd: strip width
+ * 0 = 128 B, 1 = 64 B, 2 = 256 B; xrow_fused: plan 0/1}.  Results do not depend on
+ * the variant beyond floating-point rounding of the transform order. */
+int32_t mhd_set_variants(mhd_handle* h, const int32_t* v5, int32_t* out5);
 /* Time nsteps steps with HIP events on the handle's stream (whole-step wall time). */
 int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out);
 /* Run nsteps steps with an event pair around every kernel launch and report per
@@ -187,6 +197,7 @@ int32_t mhd_batch_set_cfl(mhd_batch* b, int32_t run, double cfl_coef);
 int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps);
 int32_t mhd_batch_use_graph(mhd_batch* b, int32_t enable);
 int32_t mhd_batch_time_steps(mhd_batch* b, int32_t nsteps, double* ms_out);
+int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps);
 
 #ifdef __cplusplus
 }

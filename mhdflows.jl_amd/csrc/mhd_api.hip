@@ -223,6 +223,19 @@ int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp) {
   return h->e->set_forcing(0, seed, kf, amp);
 }
 
+int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len) {
+  MHD_TRY(check_handle(h));
+  return h->e->tune(reps, report, report_len);
+}
+int32_t mhd_set_variants(mhd_handle* h, const int32_t* v5, int32_t* out5) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_variants(v5, out5);
+}
+int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
+  for (auto& g : b->groups) MHD_TRY(g.e->tune(reps, nullptr, 0));
+  return 0;
+}
 int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out) {
   MHD_TRY(check_handle(h));
   if (!ms_out) return fail(E_INVALID, "ms_out is NULL");

@@ -26,7 +26,21 @@ struct Geom {
   int64_t nr;      // real elements per field     = nz*ny*nx
   double multx, multy, multz;  // wavenumber multipliers ((2pi/L)*n)/n
   double inv_n;    // 1/(nx*ny*nz) in Float64 (rounded to T by the kernels)
+  uint32_t fd_nkr_m, fd_nkr_l;  // division by nkr: q = (umulhi(x, m) + x) >> l for x < 2^31
+  uint32_t fd_nin_m, fd_nin_l;  // division by nkr - 2 (interior kx count)
 };
+
+// Division by an invariant integer d via multiply and shift (l = ceil(log2 d),
+// m = ceil(2^(32+l)/d) - 2^32); exact for 0 <= x < 2^31.
+inline void fastdiv_make(uint32_t d, uint32_t* m, uint32_t* l) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  *l = s;
+  *m = (uint32_t)((((1ull << (32 + s)) + d - 1) / d) - (1ull << 32));
+}
+__device__ __forceinline__ uint32_t fastdiv(uint32_t x, uint32_t m, uint32_t l) {
+  return (uint32_t)(((uint64_t)__umulhi(x, m) + x) >> l);
+}
 
 // FourierFlows getaliasedwavenumbers with aliased fraction 1/3 (as called at
 // src/utils/utils.jl:82-84): L=(1-a)/2, R=(1+a)/2, iL=floor(L n)+1, iR=ceil(R n)
@@ -58,6 +72,8 @@ inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
   g.multy = ((twopi / Ly) * ny) / ny;
   g.multz = ((twopi / Lz) * nz) / nz;
   g.inv_n = 1.0 / ((double)nx * (double)ny * (double)nz);
+  fastdiv_make((uint32_t)g.nkr, &g.fd_nkr_m, &g.fd_nkr_l);
+  fastdiv_make((uint32_t)(g.nkr - 2), &g.fd_nin_m, &g.fd_nin_l);
   return g;
 }
 

@@ -75,8 +75,8 @@ int64_t run_bytes(const Geom& g, int phys, int prec) {
   const int s = prec == 0 ? 4 : 8;
   const int nf = phys == PHYS_MHD ? 6 : 3;
   const int np = phys == PHYS_MHD ? 9 : 6;
-  const int64_t rb = prec == 0 ? (int64_t)row_rows_per_block<float>(g.nx) : (int64_t)row_rows_per_block<double>(g.nx);
-  const int64_t blocks = ((int64_t)g.ny * g.nz + rb - 1) / rb;
+  const int64_t rows = (int64_t)g.ny * g.nz;
+  const int64_t blocks = prec == 0 ? row_blocks_max<float>(g.nx, rows) : row_blocks_max<double>(g.nx, rows);
   return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s + 4096;
 }
 
@@ -151,8 +151,8 @@ class Engine : public EngineBase {
     twM = twiddles<T>(device, g.nx / 2, 0, &err);
     twh = twiddles<T>(device, g.nx, 1, &err);
     if (err) return err;
-    const int64_t rpb = row_rows_per_block<T>(g.nx);
-    row_blocks = ((int64_t)g.ny * g.nz + rpb - 1) / rpb;
+    row_blocks = row_blocks_max<T>(g.nx, (int64_t)g.ny * g.nz);  // partial capacity for any row variant
+    apply_cached_tuning();
     rbh.resize(nr_);
     force_tabs.assign(nr_, nullptr);
     for (int r = 0; r < nr_; ++r) {
@@ -223,10 +223,17 @@ class Engine : public EngineBase {
     return 0;
   }
 
-  static constexpr int S = strip_width<T>();
+  // tuned variants: col_var[c] for the hot column passes c = zpass_inv, ypass_inv,
+  // ypass_fwd, zpass_fwd (strip width), row_var for the fused row kernel (plan)
+  int col_var[4] = {0, 0, 0, 0};
+  int row_var = 0;
+
+  static int col_class(int axis, int dir) { return dir > 0 ? (axis == 2 ? 0 : 1) : (axis == 1 ? 2 : 3); }
 
   int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
-              bool prune) {
+              bool prune, bool hot = false) {
+    const int n = axis == 1 ? g.ny : g.nz;
+    const int S = col_strip<T>(n, hot ? col_var[col_class(axis, dir)] : 0);
     ColArgs a{};
     a.g = g;
     a.axis = axis;
@@ -237,7 +244,6 @@ class Engine : public EngineBase {
     a.dst_f0 = dst_f0;
     a.ncols = prune ? g.kx_keep : g.nkr;
     a.nstrips = (a.ncols + S - 1) / S;
-    const int n = axis == 1 ? g.ny : g.nz;
     if (axis == 2 && prune) {  // only pencils with kept l
       a.nother = nkept_y(g);
       a.other_lo_end = g.ly_lo_end;
@@ -259,29 +265,166 @@ class Engine : public EngineBase {
     const double cols = (double)nr_ * nfields * a.nother * a.ncols;
     const double bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
     const cx<T>* tw = axis == 1 ? twy : twz;
-    return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, rbd, tw, st); });
+    return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, S, rbd, tw, st); });
   }
 
-  RowArgs row_args() const {
+  // row launches: variant 0 everywhere except the fused products kernel (row_var)
+  RowArgs row_args(int var = 0) const {
     RowArgs r{};
     r.g = g;
     r.nruns = nr_;
     r.rows = (int64_t)g.ny * g.nz;
-    r.blocks_per_run = row_blocks;
+    const int64_t rpb = row_rows_per_block<T>(g.nx, var);
+    r.blocks_per_run = (r.rows + rpb - 1) / rpb;
     r.in_ncols = g.nkr;
     return r;
   }
 
-  int xrow(const char* name, const RowArgs& a, int kind, int nfk, double bytes) {
-    return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, rbd, twM, twh, st); });
+  int xrow(const char* name, const RowArgs& a, int kind, int nfk, double bytes, int var = 0) {
+    return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, var, rbd, twM, twh, st); });
+  }
+
+  // ---- autotuning of the hot launches ------------------------------------------
+  static std::map<std::string, std::vector<int>>& tune_cache() {
+    static std::map<std::string, std::vector<int>> c;
+    return c;
+  }
+  std::string tune_key() const {
+    return std::to_string(dev) + ":" + std::to_string(sizeof(T)) + ":" + std::to_string(g.nx) + "x" +
+           std::to_string(g.ny) + "x" + std::to_string(g.nz) + ":" + std::to_string(phys);
+  }
+  void apply_cached_tuning() {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = tune_cache().find(tune_key());
+    if (it == tune_cache().end()) return;
+    for (int c = 0; c < 4; ++c) col_var[c] = it->second[c];
+    row_var = it->second[4];
+  }
+
+  // time `reps` launches of one hot kernel with the given variant (work/stin only)
+  int time_variant(int cls, int var, int reps, float* ms) {
+    const int save_col = cls < 4 ? col_var[cls] : 0, save_row = row_var;
+    if (cls < 4) col_var[cls] = var;
+    else row_var = var;
+    hipEvent_t a, b;
+    MHD_HIP(hipEventCreate(&a));
+    MHD_HIP(hipEventCreate(&b));
+    int rc = 0;
+    for (int k = 0; k <= reps && rc == 0; ++k) {
+      if (k == 1) MHD_HIP(hipEventRecord(a, st));  // first launch warms up
+      switch (cls) {
+        case 0: rc = colpass("tune", 2, 1, B_SOL, B_WORK, 0, 0, nf, true, true); break;
+        case 1: rc = colpass("tune", 1, 1, B_WORK, B_WORK, 0, 0, nf, true, true); break;
+        case 2: rc = colpass("tune", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true); break;
+        case 3: rc = colpass("tune", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true); break;
+        default: {
+          RowArgs ra = row_args(row_var);
+          ra.in_sel = B_WORK;
+          ra.in_ncols = g.kx_keep;
+          ra.out_sel = B_WORK;
+          rc = xrow("tune", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, 0, row_var);
+        }
+      }
+    }
+    MHD_HIP(hipEventRecord(b, st));
+    MHD_HIP(hipEventSynchronize(b));
+    MHD_HIP(hipEventElapsedTime(ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (cls < 4) col_var[cls] = save_col;
+    else row_var = save_row;
+    return rc;
+  }
+
+  // Pick the fastest variant of each hot launch; the captured step is rebuilt.
+  // Uses work and stin as scratch (their contents are dead between steps).
+  int autotune(int reps, std::string* report) {
+    if (reps <= 0) reps = 5;
+    MHD_HIP(hipStreamSynchronize(st));
+    const char* names[5] = {"zpass_inv", "ypass_inv", "ypass_fwd", "zpass_fwd", "xrow_fused"};
+    std::vector<int> best(5, 0);
+    for (int cls = 0; cls < 5; ++cls) {
+      float bms = 1e30f;
+      const int nvar = cls < 4 ? 3 : 2;
+      std::string line = std::string(names[cls]) + ":";
+      for (int v = 0; v < nvar; ++v) {
+        if (cls < 4) {
+          const int n = (cls == 0 || cls == 3) ? g.nz : g.ny;
+          if (v > 0 && col_strip<T>(n, v) == col_strip<T>(n, 0)) continue;  // not a distinct shape
+        } else if (v > 0 && row_rows_per_block<T>(g.nx, v) == row_rows_per_block<T>(g.nx, 0)) {
+          continue;
+        }
+        float ms = 0;
+        MHD_TRY(time_variant(cls, v, reps, &ms));
+        line += " v" + std::to_string(v) + "=" + std::to_string(ms / reps) + "ms";
+        if (ms < bms) {
+          bms = ms;
+          best[cls] = v;
+        }
+      }
+      if (report) *report += line + " -> v" + std::to_string(best[cls]) + "\n";
+    }
+    for (int c = 0; c < 4; ++c) col_var[c] = best[c];
+    row_var = best[4];
+    {
+      std::lock_guard<std::mutex> lk(g_tab_mu);
+      tune_cache()[tune_key()] = best;
+    }
+    if (gexec) {
+      hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+    if (graph) {
+      hipGraphDestroy(graph);
+      graph = nullptr;
+    }
+    return 0;
+  }
+
+  // v5 = {zpass_inv, ypass_inv, ypass_fwd, zpass_fwd, xrow_fused}; entries < 0 keep the current choice
+  int set_variants(const int32_t* v5, int32_t* out5) override {
+    if (v5) {
+      for (int c = 0; c < 4; ++c)
+        if (v5[c] >= 0) {
+          if (v5[c] > 2) return fail(E_INVALID, "column-pass variant must be 0..2");
+          col_var[c] = v5[c];
+        }
+      if (v5[4] >= 0) {
+        if (v5[4] > 1) return fail(E_INVALID, "row variant must be 0..1");
+        row_var = v5[4];
+      }
+      if (gexec) {
+        hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+      }
+      if (graph) {
+        hipGraphDestroy(graph);
+        graph = nullptr;
+      }
+    }
+    if (out5) {
+      for (int c = 0; c < 4; ++c) out5[c] = col_var[c];
+      out5[4] = row_var;
+    }
+    return 0;
+  }
+
+  int tune(int reps, char* report, int len) override {
+    std::string r;
+    MHD_TRY(autotune(reps, &r));
+    if (report && len > 0) {
+      std::strncpy(report, r.c_str(), (size_t)len - 1);
+      report[len - 1] = 0;
+    }
+    return 0;
   }
 
   // ---- one right-hand side evaluation ----------------------------------------
   // in_sel: stage input; prune: the stage input is dealiased; last: write vars + maxima.
   int rhs(int in_sel, bool prune, SpecArgs sp, bool last) {
-    MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune));
-    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune));
-    RowArgs ra = row_args();
+    MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune, true));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune, true));
+    RowArgs ra = row_args(row_var);
     ra.in_sel = B_WORK;
     ra.in_ncols = prune ? g.kx_keep : g.nkr;
     ra.out_sel = B_WORK;
@@ -289,16 +432,17 @@ class Engine : public EngineBase {
     ra.want_max = last ? 1 : 0;
     const double rows = (double)nr_ * g.ny * g.nz;
     double bytes = rows * (nf * ra.in_ncols + np * g.nkr) * sizeof(cx<T>) + (last ? rows * nf * g.nx * sizeof(T) : 0);
-    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes));
-    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false));
-    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false));
+    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes, row_var));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true));
+    const int64_t used_blocks = ra.blocks_per_run;
     sp.g = g;
     sp.nruns = nr_;
     sp.nitems = (int64_t)nr_ * g.nc;
     sp.prod_sel = B_WORK;
     MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
     if (last) MHD_TRY(timed("reduce", 0, [&] {
-      return launch_small<T>(SK_REDUCE, rbd, nr_, phys, row_blocks, sp.mode == SP_RHS ? 0 : 1, st);
+      return launch_small<T>(SK_REDUCE, rbd, nr_, phys, used_blocks, sp.mode == SP_RHS ? 0 : 1, st);
     }));
     return 0;
   }

@@ -67,6 +67,8 @@ class EngineBase {
   virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;
   virtual int time_steps(int nsteps, double* ms) = 0;
   virtual int profile_steps(int nsteps, std::vector<KTime>* out) = 0;
+  virtual int tune(int reps, char* report, int len) = 0;
+  virtual int set_variants(const int32_t* v5, int32_t* out5) = 0;
   virtual int sync() = 0;
   virtual hipStream_t stream() const = 0;
 };

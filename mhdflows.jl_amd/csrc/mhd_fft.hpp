@@ -227,6 +227,19 @@ MHD_ROWPLAN_D(256, 4, 4, 4, 4, 4)
 MHD_ROWPLAN_D(512, 4, 4, 4, 4, 4, 2)
 #undef MHD_ROWPLAN_D
 
+// Tunable row plans: variant 1 of Float32 power-of-two rows uses four elements
+// per thread (twice the threads, half the registers); other cases have one plan.
+template <class T, int M, int VAR> struct RowPlanV : RowPlanT<T, M> {};
+template <int M> struct RowPlanV<float, M, 1> : RowPlanT<double, M> {};
+
+// elements per thread of RowPlanV, for host-side launch shapes
+inline int row_plan_E(int m, bool dbl, int var) {
+  if (m == 4) return 4;
+  if (m % 3 == 0) return 6;
+  if (dbl) return 4;
+  return var == 1 ? 4 : 8;
+}
+
 template <class L> struct RFirst;
 template <int R0, int... Rs> struct RFirst<Radices<R0, Rs...>> { static constexpr int value = R0; };
 template <class L> struct RLast;

@@ -195,12 +195,12 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
   }
 }
 
-template <class T, int NX, int KIND, int NF>
+template <class T, int NX, int KIND, int NF, int VAR>
 __global__ void __launch_bounds__(128)
 xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
             const cx<T>* __restrict__ twh) {
   constexpr int M = NX / 2;
-  using RP = RowPlanT<T, M>;
+  using RP = RowPlanV<T, M, VAR>;
   constexpr int E = RP::E;
   constexpr int TX = M / E;
   static_assert(TX <= 128, "row transform too long for one workgroup");
@@ -370,6 +370,7 @@ struct SpecArgs {
   int self_f0;
   double ca, cb;      // LSRK54 stage coefficients A_i, B_i (rounded to T in the kernel)
   int64_t nitems;     // nruns * nc
+  int planes;         // 0: interior modes 1 <= kx < nx/2; 1: the kx = 0 and kx = nx/2 planes
 };
 
 template <class T>
@@ -451,20 +452,30 @@ __global__ void __launch_bounds__(256)
 spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
   const Geom& G = a.g;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= a.nitems) return;
-  const int run = (int)(gid / G.nc);
-  const int64_t idx = gid % G.nc;
+  // grid: y = m, z = run.  Interior launch (a.planes == 0): x covers the modes
+  // 1 <= kx < nx/2 of every l.  Plane launch (a.planes == 1): x covers the kx = 0
+  // and kx = nx/2 planes, whose modes are handled in Hermitian pairs by the thread
+  // of the lower index, so the pair's stage inputs are read before either is
+  // overwritten.  Splitting keeps the interior waves free of the pair branch.
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int run = blockIdx.z;
+  const int m = blockIdx.y;
+  int l, kx;
+  if (!a.planes) {
+    const int ni = G.nkr - 2;
+    if (i >= (uint32_t)(G.ny * ni)) return;
+    l = (int)fastdiv(i, G.fd_nin_m, G.fd_nin_l);
+    kx = (int)i - l * ni + 1;
+  } else {
+    if (i >= (uint32_t)(2 * G.ny)) return;
+    const int p = i >= (uint32_t)G.ny ? 1 : 0;
+    l = (int)i - p * G.ny;
+    kx = p ? G.nx / 2 : 0;
+  }
+  const int64_t idx = ((int64_t)m * G.ny + l) * G.nkr + kx;
   const RunBuf<T>& rb = runs[run];
-  const int kx = (int)(idx % G.nkr);
-  const int64_t rest = idx / G.nkr;
-  const int l = (int)(rest % G.ny);
-  const int m = (int)(rest / G.ny);
 
-  // modes on the kx=0 and kx=nx/2 planes are handled in Hermitian pairs by the
-  // thread of the lower index, so the pair's stage inputs are read before either
-  // is overwritten.
-  const bool plane = (kx == 0 || kx == G.nx / 2);
+  const bool plane = a.planes != 0;
   int nmodes = 1;
   int64_t qs[2] = {idx, idx};
   int ls[2] = {l, l}, ms[2] = {m, m};

@@ -1,6 +1,8 @@
 // Host-side launchers: map runtime grid lengths onto the compiled kernel
 // instances.  Implemented per precision in k_launch_f32.hip / k_launch_f64.hip.
 #pragma once
+#include <algorithm>
+
 #include "mhd_kernels.hpp"
 
 namespace mhd {
@@ -24,24 +26,38 @@ template <class T> inline int col_threads(int n) {
   }
   return strip_width<T>() * (n / e);
 }
-template <class T> inline int row_rows_per_block(int nx) {
-  // must match RowPlanT<T, nx/2>::E (mhd_fft.hpp)
+template <class T> inline int row_rows_per_block(int nx, int var = 0) {
   const int m = nx / 2;
-  int e = 8;
-  const bool three = (m % 3 == 0);
-  if (m == 4) e = 4;
-  else if (three) e = 6;
-  else if (sizeof(T) == 8) e = 4;
-  else e = 8;
-  const int tx = m / e;
+  const int tx = m / row_plan_E(m, sizeof(T) == 8, var);
   return 128 / tx;
 }
+// most row blocks any variant needs (sizes the per-block partials)
+template <class T> inline int64_t row_blocks_max(int nx, int64_t rows) {
+  int64_t b = 0;
+  for (int v = 0; v < 2; ++v) {
+    const int64_t rpb = row_rows_per_block<T>(nx, v);
+    b = std::max<int64_t>(b, (rows + rpb - 1) / rpb);
+  }
+  return b;
+}
 
+// column-pass strip widths: variant 0 = one 128 B segment, 1 = half, 2 = double
+template <class T> inline int col_strip(int n, int var) {
+  const int sw = strip_width<T>();
+  const int s = var == 1 ? sw / 2 : var == 2 ? sw * 2 : sw;
+  const int e = (n == 8 || n == 16 || n == 32 || n == 64 || n == 512) ? 8 : (n % 3 == 0) ? 12 : 16;
+  const int threads = s * (n / e);
+  const int64_t lds = (int64_t)n * s * (int64_t)sizeof(cx<T>);
+  if (threads > 1024 || lds > 160 * 1024) return sw;
+  return s;
+}
+
+// S = col_strip<T>(n, var) columns per tile (ColArgs.nstrips must use the same S)
 template <class T>
-hipError_t launch_colpass(const ColArgs& a, int n, int dir, const RunBuf<T>* runs, const cx<T>* tw,
+hipError_t launch_colpass(const ColArgs& a, int n, int dir, int S, const RunBuf<T>* runs, const cx<T>* tw,
                           hipStream_t s);
 template <class T>
-hipError_t launch_xrow(const RowArgs& a, int kind, int nf, const RunBuf<T>* runs, const cx<T>* twM,
+hipError_t launch_xrow(const RowArgs& a, int kind, int nf, int var, const RunBuf<T>* runs, const cx<T>* twM,
                        const cx<T>* twh, hipStream_t s);
 template <class T>
 hipError_t launch_spectral(const SpecArgs& a, int phys, const RunBuf<T>* runs, hipStream_t s);

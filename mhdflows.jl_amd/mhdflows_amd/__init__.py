@@ -13,6 +13,8 @@ from .problem import (  # noqa: F401
     Problem,
     SetUpProblemIC,
     SetUpProblemIC_modes,
+    autotune,
+    variants,
     TimeIntegrator,
     UiUpdate,
     calcN,

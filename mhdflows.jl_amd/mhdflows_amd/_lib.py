@@ -76,6 +76,9 @@ SIGNATURES = {
     "mhd_refresh_vars": (_I, [_P]),
     "mhd_set_forcing": (_I, [_P, C.c_uint64, C.c_double, C.c_double]),
     "mhd_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
+    "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),
+    "mhd_set_variants": (_I, [_P, _P, _P]),
+    "mhd_batch_autotune": (_I, [_P, _I]),
     "mhd_profile_steps": (_I, [_P, _I, C.POINTER(KTime), _I, C.POINTER(_I)]),
     "mhd_step_bytes_model": (C.c_double, [C.POINTER(Config)]),
     "mhd_batch_create": (_I, [C.POINTER(Config), _I, C.POINTER(_P)]),

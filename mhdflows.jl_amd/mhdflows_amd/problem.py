@@ -367,5 +367,22 @@ def set_forcing(prob, seed, kf=2.0, amp=1.0):
     check(lib().mhd_set_forcing(prob._h, int(seed), float(kf), float(amp)))
 
 
+def autotune(prob, reps=5):
+    """Pick the fastest compiled variant of each hot launch for this grid (see mhd_autotune)."""
+    buf = C.create_string_buffer(4096)
+    check(lib().mhd_autotune(prob._h, int(reps), buf, len(buf)))
+    return buf.value.decode()
+
+
+def variants(prob, set_to=None):
+    """Read (and optionally set) the launch variants [zinv, yinv, yfwd, zfwd, row]."""
+    out = (C.c_int32 * 5)()
+    src = None
+    if set_to is not None:
+        src = (C.c_int32 * 5)(*[int(v) for v in set_to])
+    check(lib().mhd_set_variants(prob._h, src, out))
+    return list(out)
+
+
 def refresh_vars(prob):
     check(lib().mhd_refresh_vars(prob._h))

@@ -290,6 +290,27 @@ def test_rhs_parity_fp32_three_factor(n):
     assert rel_l2(N_gpu, N_ref) < 2e-5
 
 
+@pytest.mark.parametrize("var", [[1, 1, 1, 1, 1], [2, 2, 2, 2, 0], [0, 2, 1, 0, 1]])
+@pytest.mark.parametrize("T,n,tol", [(np.float32, 64, 2e-5), (np.float64, 32, 1e-12)])
+def test_variants_parity(var, T, n, tol):
+    say("variants", var, T.__name__, n)
+    o, p = pair(n, T=T, kmax=n / 2.0)
+    assert M.variants(p, var) == [v for v in var]
+    run_steps(o, 2)
+    M.stepforward(p, 2)
+    assert rel_l2(p.sol, o.sol) < tol
+
+
+def test_autotune_keeps_results():
+    say("autotune")
+    o, p = pair(64, T=np.float64, kmax=10.0)
+    rep = M.autotune(p, 2)
+    assert "xrow_fused" in rep
+    run_steps(o, 2)
+    M.stepforward(p, 2)
+    assert rel_l2(p.sol, o.sol) < 1e-12
+
+
 def test_fp32_rhs_128_against_oracle():
     say("rhs fp32 128")
     o, p = pair(128, T=np.float32, kmax=40.0, mode="fused")
