@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--variants", type=str, default="",
+                    help="fixed launch variants zinv,yinv,yfwd,zfwd,row instead of autotuning")
+    ap.add_argument("--no-copy", action="store_true", help="skip the reference copy-bandwidth measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (single-GPU build; "
@@ -96,10 +99,13 @@ def main():
     import ctypes as C
     step_bytes = M.lib().mhd_step_bytes_model(C.byref(cfg))
 
-    log("bench: autotuning launch variants")
-    tune_report = M.autotune(p, 5)
-    log(tune_report.strip())
-    chosen = M.variants(p)
+    if args.variants:
+        chosen = M.variants(p, [int(v) for v in args.variants.split(",")])
+    else:
+        log("bench: autotuning launch variants")
+        tune_report = M.autotune(p, 5)
+        log(tune_report.strip())
+        chosen = M.variants(p)
 
     log(f"bench: warmup {args.warmup} steps")
     M.TimeIntegrator(p, math.inf, max(args.warmup, 1) - 1, CFL_Coef=0.3)
@@ -131,6 +137,14 @@ def main():
                            "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
                            if k["bytes"] > 0 else None} for k in prof}
 
+    copy_gbs = None
+    if not args.no_copy:
+        ms = C.c_double()
+        nbytes = 1 << 30
+        M._lib.check(M.lib().mhd_bench_copy(nbytes, 20, C.byref(ms)))
+        copy_gbs = round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)
+        log(f"bench: reference copy kernel {copy_gbs} GB/s (read+write)")
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, literal structure)")
@@ -156,6 +170,7 @@ def main():
         "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "copy_kernel_GBps": copy_gbs,
         "variants": {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
                      "zpass_fwd": chosen[3], "xrow_fused": chosen[4]},
         "cpu_baseline": cpu,

@@ -176,6 +176,10 @@ int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out);
 /* Run nsteps steps with an event pair around every kernel launch and report per
  * kernel class launches, time and algorithmic bytes (at most max entries). */
 int32_t mhd_profile_steps(mhd_handle* h, int32_t nsteps, mhd_ktime* out, int32_t max, int32_t* n_out);
+/* Reference streaming copy of `bytes` bytes (16 B per lane), timed over reps launches:
+ * average ms per launch.  Used to calibrate HBM byte counters and as the practical
+ * bandwidth ceiling next to the 8 TB/s peak. */
+int32_t mhd_bench_copy(uint64_t bytes, int32_t reps, double* ms_out);
 /* Algorithmic bytes of one step (the 3-pass model: MHD RK4 = 438 V_c + 120 V_r). */
 double mhd_step_bytes_model(const mhd_config* cfg);
 
@@ -197,6 +201,18 @@ int32_t mhd_batch_set_cfl(mhd_batch* b, int32_t run, double cfl_coef);
 int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps);
 int32_t mhd_batch_use_graph(mhd_batch* b, int32_t enable);
 int32_t mhd_batch_time_steps(mhd_batch* b, int32_t nsteps, double* ms_out);
+/* Serving-pool control.  A run of a batch is a slot: set its physical parameters,
+ * then start it with a fresh clock (t = 0, step = 0), a dt (or CFL coefficient > 0 for
+ * getCFL! before every step) and a step target; step launches skip a run once its
+ * step count reaches the target, so slots join and retire at step boundaries while
+ * the captured ensemble step stays the same.  New batch runs start with no target. */
+int32_t mhd_batch_set_params(mhd_batch* b, int32_t run, double nu, double eta, int32_t n_nu);
+int32_t mhd_batch_start_run(mhd_batch* b, int32_t run, double dt, double cfl_coef, int64_t target_steps);
+/* Enqueue nsteps steps of every run without waiting; query reports 1 while work is
+ * pending; sync waits for it. */
+int32_t mhd_batch_launch(mhd_batch* b, int32_t nsteps);
+int32_t mhd_batch_query(mhd_batch* b, int32_t* busy);
+int32_t mhd_batch_sync(mhd_batch* b);
 int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps);
 
 #ifdef __cplusplus

@@ -10,6 +10,13 @@
 
 using namespace mhd;
 
+// plain streaming copy (16 B per lane), the reference kernel for HBM byte counters
+__global__ void __launch_bounds__(256)
+copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
 struct mhd_handle {
   mhd_config cfg;
   EngineBase* e;
@@ -236,6 +243,32 @@ int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps) {
   for (auto& g : b->groups) MHD_TRY(g.e->tune(reps, nullptr, 0));
   return 0;
 }
+int32_t mhd_bench_copy(uint64_t bytes, int32_t reps, double* ms_out) {
+  if (bytes < 16 || reps <= 0 || !ms_out) return fail(E_INVALID, "bytes >= 16 and reps > 0 required");
+  const int64_t n16 = (int64_t)(bytes / 16);
+  void *a = nullptr, *b = nullptr;
+  MHD_HIP(hipMalloc(&a, n16 * 16));
+  MHD_HIP(hipMalloc(&b, n16 * 16));
+  MHD_HIP(hipMemset(a, 1, n16 * 16));
+  hipEvent_t e0, e1;
+  MHD_HIP(hipEventCreate(&e0));
+  MHD_HIP(hipEventCreate(&e1));
+  const unsigned nb = 256 * 32;
+  hipLaunchKernelGGL(copy16_kernel, dim3(nb), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n16);
+  MHD_HIP(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(copy16_kernel, dim3(nb), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n16);
+  MHD_HIP(hipEventRecord(e1, 0));
+  MHD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MHD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return 0;
+}
 int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out) {
   MHD_TRY(check_handle(h));
   if (!ms_out) return fail(E_INVALID, "ms_out is NULL");
@@ -347,6 +380,36 @@ int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps) {
   if (!b) return fail(E_INVALID, "batch is NULL");
   // every group steps on its own stream; the groups overlap on the device
   for (auto& g : b->groups) MHD_TRY(g.e->step(nsteps));
+  for (auto& g : b->groups) MHD_TRY(g.e->sync());
+  return 0;
+}
+int32_t mhd_batch_set_params(mhd_batch* b, int32_t run, double nu, double eta, int32_t n_nu) {
+  BATCH_RUN(b, run);
+  if (nu < 0 || eta < 0) return fail(E_INVALID, "viscosity and resistivity must be >= 0");
+  return G_.e->set_run_params(L_, nu, eta, n_nu);
+}
+int32_t mhd_batch_start_run(mhd_batch* b, int32_t run, double dt, double cfl_coef, int64_t target_steps) {
+  BATCH_RUN(b, run);
+  if (target_steps < 0) return fail(E_INVALID, "target_steps must be >= 0");
+  return G_.e->start_run(L_, dt, cfl_coef, target_steps);
+}
+int32_t mhd_batch_launch(mhd_batch* b, int32_t nsteps) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
+  for (auto& g : b->groups) MHD_TRY(g.e->step(nsteps));
+  return 0;
+}
+int32_t mhd_batch_query(mhd_batch* b, int32_t* busy) {
+  if (!b || !busy) return fail(E_INVALID, "batch or busy is NULL");
+  *busy = 0;
+  for (auto& g : b->groups) {
+    const int q = g.e->query();
+    if (q < 0) return q;
+    *busy |= q;
+  }
+  return 0;
+}
+int32_t mhd_batch_sync(mhd_batch* b) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
   for (auto& g : b->groups) MHD_TRY(g.e->sync());
   return 0;
 }

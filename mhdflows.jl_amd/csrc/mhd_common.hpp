@@ -96,7 +96,15 @@ struct RunClock {
   double dl;         // min(dx,dy,dz) in T
   int32_t use_cfl;   // 1: dt from getCFL! before each step
   int32_t nan_flag;  // set when the maxima are NaN
+  int64_t target;    // step launches skip this run once step >= target (pool slots)
 };
+
+// Step launches of a run whose clock reached its target do nothing (finished or
+// empty slots of a serving pool); the step counter only moves at the end of a step.
+template <class RB>
+__device__ __forceinline__ bool gated_off(int gate, const RB& rb) {
+  return gate && rb.clk->step >= rb.clk->target;
+}
 
 // Per-run descriptor read by every kernel (one per run of a batched launch).
 template <class T>

@@ -189,6 +189,7 @@ class Engine : public EngineBase {
       c.dt = (double)(T)rp[r].dt;
       c.dl = (double)std::min({clk_dx(0), clk_dx(1), clk_dx(2)});
       c.t_diff = std::numeric_limits<double>::infinity();
+      c.target = std::numeric_limits<int64_t>::max();
       clk_init.push_back(c);
       MHD_HIP(hipMemcpyAsync(b.clk, &c, sizeof(c), hipMemcpyHostToDevice, st));
     }
@@ -230,6 +231,25 @@ class Engine : public EngineBase {
 
   static int col_class(int axis, int dir) { return dir > 0 ? (axis == 2 ? 0 : 1) : (axis == 1 ? 2 : 3); }
 
+  int gate_ = 0;  // set while a time step is being launched
+
+  // Launch scope: helper launches cover runs [run0_, run0_ + cur_n()) of the group
+  // (all runs unless a RunScope narrows them to one, e.g. to set up a single slot).
+  int run0_ = 0, nrun_ = -1;
+  int cur_n() const { return nrun_ < 0 ? nr_ : nrun_; }
+  RunBuf<T>* cur_rb() const { return rbd + run0_; }
+  struct RunScope {
+    Engine* e;
+    RunScope(Engine* en, int r) : e(en) {
+      e->run0_ = r;
+      e->nrun_ = 1;
+    }
+    ~RunScope() {
+      e->run0_ = 0;
+      e->nrun_ = -1;
+    }
+  };
+
   int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
               bool prune, bool hot = false) {
     const int n = axis == 1 ? g.ny : g.nz;
@@ -259,29 +279,31 @@ class Engine : public EngineBase {
     } else {
       a.mask_lo = a.mask_hi = 0;
     }
-    a.nruns = nr_;
-    a.nitems = (int64_t)nr_ * nfields * a.nother * a.nstrips;
+    a.nruns = cur_n();
+    a.nitems = (int64_t)cur_n() * nfields * a.nother * a.nstrips;
+    a.gate = gate_;
     const double cxb = sizeof(cx<T>);
-    const double cols = (double)nr_ * nfields * a.nother * a.ncols;
+    const double cols = (double)cur_n() * nfields * a.nother * a.ncols;
     const double bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
     const cx<T>* tw = axis == 1 ? twy : twz;
-    return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, S, rbd, tw, st); });
+    return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, S, cur_rb(), tw, st); });
   }
 
   // row launches: variant 0 everywhere except the fused products kernel (row_var)
   RowArgs row_args(int var = 0) const {
     RowArgs r{};
     r.g = g;
-    r.nruns = nr_;
+    r.nruns = cur_n();
     r.rows = (int64_t)g.ny * g.nz;
     const int64_t rpb = row_rows_per_block<T>(g.nx, var);
     r.blocks_per_run = (r.rows + rpb - 1) / rpb;
     r.in_ncols = g.nkr;
+    r.gate = gate_;
     return r;
   }
 
   int xrow(const char* name, const RowArgs& a, int kind, int nfk, double bytes, int var = 0) {
-    return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, var, rbd, twM, twh, st); });
+    return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, var, cur_rb(), twM, twh, st); });
   }
 
   // ---- autotuning of the hot launches ------------------------------------------
@@ -440,6 +462,7 @@ class Engine : public EngineBase {
     sp.nruns = nr_;
     sp.nitems = (int64_t)nr_ * g.nc;
     sp.prod_sel = B_WORK;
+    sp.gate = gate_;
     MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
     if (last) MHD_TRY(timed("reduce", 0, [&] {
       return launch_small<T>(SK_REDUCE, rbd, nr_, phys, used_blocks, sp.mode == SP_RHS ? 0 : 1, st);
@@ -479,6 +502,13 @@ class Engine : public EngineBase {
 
   // ---- one time step -----------------------------------------------------------
   int step_launches() {
+    gate_ = 1;
+    int rc = step_launches_gated();
+    gate_ = 0;
+    return rc;
+  }
+
+  int step_launches_gated() {
     MHD_TRY(timed("cfl", 0, [&] { return launch_small<T>(SK_CFL, rbd, nr_, phys, 0, 0, st); }));
     if (stepper == STEP_RK4) {
       for (int s = 0; s < 4; ++s) {
@@ -555,6 +585,7 @@ class Engine : public EngineBase {
       MHD_HIP(hipMemcpyAsync(b.vars + (int64_t)f * g.nr, fields[f], sizeof(T) * g.nr, hipMemcpyHostToDevice, st));
     }
     if (!any) return 0;
+    RunScope scope(this, run);  // only this run's buffers are transformed
     MHD_TRY(r2c_vars_to_work(run));
     for (int f = 0; f < nf; ++f) {
       if (!fields[f]) continue;
@@ -584,8 +615,8 @@ class Engine : public EngineBase {
   // max u^2 / b^2 over vars into the clocks (getCFL! input after the fields are set)
   int vars_maxima() {
     const int64_t nb = std::min<int64_t>(row_blocks, 1024);
-    MHD_TRY(timed("varsmax", 0, [&] { return launch_small<T>(SK_VARSMAX, rbd, nr_, nf, nb, (int)g.nr, st); }));
-    MHD_TRY(timed("reduce", 0, [&] { return launch_small<T>(SK_REDUCE, rbd, nr_, phys, nb, 0, st); }));
+    MHD_TRY(timed("varsmax", 0, [&] { return launch_small<T>(SK_VARSMAX, cur_rb(), cur_n(), nf, nb, (int)g.nr, st); }));
+    MHD_TRY(timed("reduce", 0, [&] { return launch_small<T>(SK_REDUCE, cur_rb(), cur_n(), phys, nb, 0, st); }));
     return 0;
   }
 
@@ -669,6 +700,34 @@ class Engine : public EngineBase {
     c.dt = (double)(T)dt;
     c.step = step;
     return write_clock(run, c);
+  }
+  int set_run_params(int run, double nu, double eta, int nnu) override {
+    MHD_TRY(check_run(run));
+    rbh[run].nu = nu;
+    rbh[run].eta = phys == PHYS_MHD ? eta : 0.0;
+    rbh[run].nnu = nnu;
+    MHD_TRY(upload_runs());
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int start_run(int run, double dt, double cfl_coef, int64_t target) override {
+    MHD_TRY(check_run(run));
+    RunClock c;
+    MHD_TRY(read_clock(run, &c));
+    c.t = 0.0;
+    c.step = 0;
+    c.dt = (double)(T)dt;
+    c.use_cfl = cfl_coef > 0 ? 1 : 0;
+    c.cfl_coef = cfl_coef;
+    c.t_diff = t_diff(run, cfl_coef > 0 ? cfl_coef : 1.0);
+    c.target = target;
+    return write_clock(run, c);
+  }
+  int query() override {
+    hipError_t e = hipStreamQuery(st);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return fail(E_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
   }
   int set_cfl(int run, int use, double coef) override {
     MHD_TRY(check_run(run));
@@ -811,7 +870,10 @@ class Engine : public EngineBase {
   }
 
   // ---- vars refresh and diagnostics -------------------------------------------
-  int refresh_vars_run(int /*run*/) { return refresh_vars(); }
+  int refresh_vars_run(int run) {
+    RunScope scope(this, run);
+    return refresh_vars();
+  }
 
   int refresh_vars() override {
     MHD_TRY(colpass("zpass_inv", 2, 1, B_SOL, B_WORK, 0, 0, nf, true));

@@ -68,6 +68,10 @@ class EngineBase {
   virtual int time_steps(int nsteps, double* ms) = 0;
   virtual int profile_steps(int nsteps, std::vector<KTime>* out) = 0;
   virtual int tune(int reps, char* report, int len) = 0;
+  // serving-pool slot control: physical parameters, and a fresh clock with a step target
+  virtual int set_run_params(int run, double nu, double eta, int nnu) = 0;
+  virtual int start_run(int run, double dt, double cfl_coef, int64_t target) = 0;
+  virtual int query() = 0;  // 1 while the stream has work, 0 when idle
   virtual int set_variants(const int32_t* v5, int32_t* out5) = 0;
   virtual int sync() = 0;
   virtual hipStream_t stream() const = 0;

@@ -254,20 +254,37 @@ template <int R0, int... Rs, int... As> struct RReverse<Radices<R0, Rs...>, Radi
 };
 
 // ---------------------------------------------------------------------------
+// Synchronisation of an LDS exchange.  WS = true when all threads of a transform
+// belong to one wavefront: a wavefront executes its LDS operations in order, so a
+// wavefront-scope release/acquire fence around a wave barrier (no cross-wave
+// waiting) orders the exchange; otherwise a workgroup barrier is used.
+// ---------------------------------------------------------------------------
+template <bool WS>
+__device__ __forceinline__ void xsync() {
+  if constexpr (WS) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The pass pipeline.  x[E] holds the thread's values in the input layout of the
 // first pass on entry and in the output layout of the last pass on exit.
 // Lds(pos) returns a reference to the exchange slot of position pos of this
-// thread's transform.  A barrier precedes every exchange write and follows every
+// thread's transform.  A sync precedes every exchange write and follows every
 // exchange read, so the same LDS area may be reused by the caller afterwards.
 // ---------------------------------------------------------------------------
-template <class T, int N, int TH, int E, int S, int Ns, class Rad> struct Passes;
+template <class T, int N, int TH, int E, int S, int Ns, bool WS, class Rad> struct Passes;
 
-template <class T, int N, int TH, int E, int S, int Ns> struct Passes<T, N, TH, E, S, Ns, Radices<>> {
+template <class T, int N, int TH, int E, int S, int Ns, bool WS> struct Passes<T, N, TH, E, S, Ns, WS, Radices<>> {
   template <class Lds> static __device__ __forceinline__ void run(cx<T>*, int, Lds&, const cx<T>*) {}
 };
 
-template <class T, int N, int TH, int E, int S, int Ns, int R, int... Rest>
-struct Passes<T, N, TH, E, S, Ns, Radices<R, Rest...>> {
+template <class T, int N, int TH, int E, int S, int Ns, bool WS, int R, int... Rest>
+struct Passes<T, N, TH, E, S, Ns, WS, Radices<R, Rest...>> {
   template <class Lds>
   static __device__ __forceinline__ void run(cx<T>* x, int t, Lds& lds, const cx<T>* __restrict__ tw) {
     static_assert(E % R == 0, "radix must divide elements per thread");
@@ -286,7 +303,7 @@ struct Passes<T, N, TH, E, S, Ns, Radices<R, Rest...>> {
     }
     if constexpr (sizeof...(Rest) > 0) {
       constexpr int R2 = RFirst<Radices<Rest...>>::value;
-      __syncthreads();
+      xsync<WS>();
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
         const int b = t + i * TH;
@@ -294,21 +311,21 @@ struct Passes<T, N, TH, E, S, Ns, Radices<R, Rest...>> {
 #pragma unroll
         for (int r = 0; r < R; ++r) lds(j + r * Ns) = x[i * R + r];
       }
-      __syncthreads();
+      xsync<WS>();
 #pragma unroll
       for (int i = 0; i < E / R2; ++i) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) x[i * R2 + r] = lds(t + i * TH + r * (N / R2));
       }
-      Passes<T, N, TH, E, S, Ns * R, Radices<Rest...>>::run(x, t, lds, tw);
+      Passes<T, N, TH, E, S, Ns * R, WS, Radices<Rest...>>::run(x, t, lds, tw);
     }
   }
 };
 
 // Whole transform: x in the pass-0 input layout, returns the last-pass output layout.
-template <class T, int N, int E, int S, class Rad, class Lds>
+template <class T, int N, int E, int S, class Rad, bool WS = false, class Lds>
 __device__ __forceinline__ void fft_regs(cx<T>* x, int t, Lds& lds, const cx<T>* __restrict__ tw) {
-  Passes<T, N, N / E, E, S, 1, Rad>::run(x, t, lds, tw);
+  Passes<T, N, N / E, E, S, 1, WS, Rad>::run(x, t, lds, tw);
 }
 
 }  // namespace mhd

@@ -41,6 +41,7 @@ struct ColArgs {
   int mask_lo, mask_hi;  // input positions in [mask_lo, mask_hi) are zero (not read)
   int nruns;
   int64_t nitems;
+  int gate;            // 1 inside a time step: skip runs whose clock reached its target
 };
 
 template <class T, int N, int S, int DIR>
@@ -71,6 +72,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const int run = (int)(it / a.nfields);
   const int o = oi < a.other_lo_end ? oi : oi + a.other_gap;
   const RunBuf<T>& rb = runs[run];
+  if (gated_off(a.gate, rb)) return;
   const int col = strip * S + c;
   const bool col_ok = col < a.ncols;
   const int64_t base = (int64_t)o * ostride + col;
@@ -128,6 +130,7 @@ struct RowArgs {
   int want_max;       // c2r kinds: per-block max of u^2 and b^2 into partial
   int64_t rows;       // ny*nz
   int64_t blocks_per_run;
+  int gate;           // 1 inside a time step: skip runs whose clock reached its target
 };
 
 // inputs, products and self-transforms (r2c of the inputs) of each row kind
@@ -173,18 +176,18 @@ __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b
 
 // Forward real transform of the values z (one complex per pair of real samples,
 // in the pass-0 layout of Rfwd) and store the nkr half-spectrum entries.
-template <class T, int M, int E, int TX, class Rfwd, class Lds>
+template <class T, int M, int E, int TX, class Rfwd, bool WS, class Lds>
 __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>* xrow,
                                               const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twh,
                                               cx<T>* __restrict__ dst, bool row_ok) {
   constexpr int RfL = RLast<Rfwd>::value;
-  fft_regs<T, M, E, -1, Rfwd>(z, tr, ldsf, twM);
-  __syncthreads();
+  fft_regs<T, M, E, -1, Rfwd, WS>(z, tr, ldsf, twM);
+  xsync<WS>();
 #pragma unroll
   for (int i = 0; i < E / RfL; ++i)
 #pragma unroll
     for (int r = 0; r < RfL; ++r) xrow[tr + i * TX + r * (M / RfL)] = z[i * RfL + r];
-  __syncthreads();
+  xsync<WS>();
   for (int k = tr; k <= M; k += TX) {
     const cx<T> Zk = xrow[k == M ? 0 : k];
     const cx<T> Zm = conj(xrow[k == 0 ? 0 : M - k]);
@@ -205,6 +208,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   constexpr int TX = M / E;
   static_assert(TX <= 128, "row transform too long for one workgroup");
   constexpr int RPW = 128 / TX;
+  // a row's TX threads are lanes of one wavefront when TX <= 64 (TX is a power of
+  // two): its LDS exchanges then need only wavefront-scope ordering
+  static_assert((TX & (TX - 1)) == 0, "threads per row must be a power of two");
+  constexpr bool WS = TX <= 64;
   using Rinv = typename RP::R;
   using Rfwd = typename RReverse<Rinv>::type;
   constexpr int R0 = RFirst<Rinv>::value;   // first inverse radix == last forward radix
@@ -228,6 +235,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   const int64_t row = blk * RPW + rw;
   const bool row_ok = row < a.rows;
   const RunBuf<T>& rb = runs[run];
+  if (gated_off(a.gate, rb)) return;
   T* const rbase = a.real_sel ? rb.rscr : rb.vars;
   cx<T>* xrow = xb[rw];
   auto ldsf = [&](int pos) -> cx<T>& { return xrow[pos]; };
@@ -241,10 +249,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
       const cx<T>* __restrict__ src = rb.buf[a.in_sel] + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
-      __syncthreads();
+      xsync<WS>();
       for (int k = tr; k <= M; k += TX)
         xrow[k] = (row_ok && k < a.in_ncols) ? src[k] : cx<T>{T(0), T(0)};
-      __syncthreads();
+      xsync<WS>();
 #pragma unroll
       for (int i = 0; i < E / R0; ++i) {
 #pragma unroll
@@ -258,7 +266,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
           v[f][i * R0 + r] = A + mul_si<1>(cmulc(B, twh[p]));
         }
       }
-      fft_regs<T, M, E, 1, Rinv>(v[f], tr, ldsf, twM);
+      fft_regs<T, M, E, 1, Rinv, WS>(v[f], tr, ldsf, twM);
 #pragma unroll
       for (int e = 0; e < E; ++e) v[f][e] = v[f][e] * scale;
     }
@@ -343,7 +351,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       cx<T>* __restrict__ dst =
           p < NPROD ? rb.buf[a.out_sel] + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
                     : rb.buf[a.out2_sel] + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
-      row_r2c_store<T, M, E, TX, Rfwd>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
+      row_r2c_store<T, M, E, TX, Rfwd, WS>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
     }
   }
 }
@@ -371,6 +379,7 @@ struct SpecArgs {
   double ca, cb;      // LSRK54 stage coefficients A_i, B_i (rounded to T in the kernel)
   int64_t nitems;     // nruns * nc
   int planes;         // 0: interior modes 1 <= kx < nx/2; 1: the kx = 0 and kx = nx/2 planes
+  int gate;           // 1 inside a time step: skip runs whose clock reached its target
 };
 
 template <class T>
@@ -474,6 +483,7 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   }
   const int64_t idx = ((int64_t)m * G.ny + l) * G.nkr + kx;
   const RunBuf<T>& rb = runs[run];
+  if (gated_off(a.gate, rb)) return;
 
   const bool plane = a.planes != 0;
   int nmodes = 1;
@@ -590,7 +600,7 @@ __global__ void cfl_kernel(const RunBuf<T>* __restrict__ runs, int nruns, int ph
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nruns) return;
   RunClock* c = runs[r].clk;
-  if (!c->use_cfl) return;
+  if (!c->use_cfl || c->step >= c->target) return;
   T v = sqrt(T(c->maxu2));
   if (phys == PHYS_MHD) {
     const T va = sqrt(T(c->maxb2));
@@ -607,6 +617,7 @@ __global__ void __launch_bounds__(256)
 reduce_max_kernel(const RunBuf<T>* __restrict__ runs, int64_t nblocks, int advance) {
   __shared__ T red[2][256];
   const RunBuf<T>& rb = runs[blockIdx.x];
+  if (gated_off(advance, rb)) return;
   const int t = threadIdx.x;
   T mu = T(0), mb = T(0);
   for (int64_t i = t; i < nblocks; i += 256) {
@@ -671,6 +682,7 @@ __global__ void advance_clock_kernel(const RunBuf<T>* __restrict__ runs, int nru
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nruns) return;
   RunClock* c = runs[r].clk;
+  if (c->step >= c->target) return;
   c->t = (double)(T(c->t) + T(c->dt));
   c->step += 1;
 }

@@ -81,6 +81,7 @@ SIGNATURES = {
     "mhd_batch_autotune": (_I, [_P, _I]),
     "mhd_profile_steps": (_I, [_P, _I, C.POINTER(KTime), _I, C.POINTER(_I)]),
     "mhd_step_bytes_model": (C.c_double, [C.POINTER(Config)]),
+    "mhd_bench_copy": (_I, [C.c_uint64, _I, C.POINTER(C.c_double)]),
     "mhd_batch_create": (_I, [C.POINTER(Config), _I, C.POINTER(_P)]),
     "mhd_batch_destroy": (_I, [_P]),
     "mhd_batch_set_ic_real": (_I, [_P, _I, C.POINTER(_P)]),
@@ -91,6 +92,11 @@ SIGNATURES = {
     "mhd_batch_step": (_I, [_P, _I]),
     "mhd_batch_use_graph": (_I, [_P, _I]),
     "mhd_batch_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
+    "mhd_batch_set_params": (_I, [_P, _I, C.c_double, C.c_double, _I]),
+    "mhd_batch_start_run": (_I, [_P, _I, C.c_double, C.c_double, C.c_int64]),
+    "mhd_batch_launch": (_I, [_P, _I]),
+    "mhd_batch_query": (_I, [_P, C.POINTER(_I)]),
+    "mhd_batch_sync": (_I, [_P]),
 }
 
 _lib = None
