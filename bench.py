@@ -69,6 +69,103 @@ def cpu_baseline(fields, n, steps=1):
                       f"single-threaded elementwise), {dt:.1f} s"}
 
 
+def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
+    """The 1024^3 Float32 MHD RK4 step (nu = eta = 2e-4, CFL 0.3, seeded |k| <= 16 fields
+    set as sparse modes on the device), steps/s and the step roofline."""
+    import ctypes as C
+    import numpy as np
+    from mhdflows_amd import ic
+    from mhdflows_amd.devmem import hip
+    try:
+        p = M.Problem("MI355X", nx=n, ν=2e-4, η=2e-4, B_field=True, T=np.float32, stepper="RK4")
+    except M.MHDError as e:
+        return {"grid": n, "error": str(e)}
+    idx, vals = ic.solenoidal_modes(2024, k0=4.0, kmax=16.0).half_entries(n)
+    M.SetUpProblemIC_modes(p, idx, vals)
+    if variants:
+        M.variants(p, variants)
+    else:
+        M.autotune(p, 2)
+    M.TimeIntegrator(p, math.inf, warmup - 1, CFL_Coef=0.3)
+    hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
+    hip().hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    step_bytes = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
+    ms = wall * 1e3 / steps
+    gbs = step_bytes / (ms * 1e-3) / 1e9
+    d = M.Diagnostics(p)
+    out = {"grid": n, "steps": steps, "value": round(steps / wall, 4), "unit": "steps/s",
+           "ms_per_step": round(ms, 2), "variants": M.variants(p),
+           "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(gbs, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
+           "finite": not d["has_nan"]}
+    del p
+    return out
+
+
+def ensemble_bench(M, steps=20):
+    """64 runs (16 each of 32^3, 48^3, 64^3, 96^3), Float32, LSRK54, CFL 0.3, seeded
+    k^4 exp(-2(k/k0)^2) fields with k0 ~ U[2,6], nu = eta log-uniform in [1e-3, 1e-2]:
+    aggregate run-steps/s of the ensemble vs the same runs stepped one after another."""
+    import numpy as np
+    from mhdflows_amd import ensemble as E
+    from mhdflows_amd import ic
+    rng = np.random.Generator(np.random.PCG64(4))
+    specs = []
+    for n in (32, 48, 64, 96):
+        for _ in range(16):
+            specs.append((n, float(rng.uniform(2, 6)), float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-2)))),
+                          int(rng.integers(1 << 30))))
+    modes = [ic.solenoidal_modes(s, k0=k0, kmax=min(16.0, n // 3 - 1)).half_entries(n) for (n, k0, nu, s) in specs]
+    cfgs = [E.make_config(n, nu=nu, eta=nu, T=np.float32, stepper="LSRK54") for (n, k0, nu, s) in specs]
+    ens = E.Ensemble(cfgs)
+    for r, (idx, vals) in enumerate(modes):
+        ens.set_ic_modes(r, idx, vals)
+        ens.set_cfl(r, 0.3)
+    ens.autotune(3)
+    ens.step(2)
+    ens.sync()
+    ms = ens.time_steps(steps)
+    ens_rate = len(specs) * steps / (ms * 1e-3)
+    ens.close()
+    # sequential plain path: each run alone (own handle, captured step), one after another
+    from mhdflows_amd.devmem import hip
+    total = 0.0
+    for (n, k0, nu, s), (idx, vals) in zip(specs, modes):
+        p = M.Problem("MI355X", nx=n, ν=nu, η=nu, B_field=True, T=np.float32, stepper="LSRK54")
+        M.SetUpProblemIC_modes(p, idx, vals)
+        M.TimeIntegrator(p, math.inf, 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        total += time.perf_counter() - t0
+        del p
+    seq_rate = len(specs) * steps / total
+    return {"workload": "64 runs: 16 each of 32^3, 48^3, 64^3, 96^3, Float32 LSRK54, CFL 0.3",
+            "steps": steps, "ensemble_run_steps_per_s": round(ens_rate, 1),
+            "sequential_run_steps_per_s": round(seq_rate, 1), "speedup": round(ens_rate / seq_rate, 2)}
+
+
+def serving_bench(nreq, rate, steps, seed=7):
+    """Poisson stream of MHD solve requests (grid 32/64/128/256 with weights .4/.3/.2/.1,
+    Float32/Float64 90/10, nu = eta log-uniform in [1e-4, 1e-2], seeded k^4 exp(-2(k/4)^2)
+    fields, CFL 0.3), admitted by an HBM budget into slot pools."""
+    import numpy as np
+    from mhdflows_amd import serve
+    reqs = serve.poisson_stream(nreq, rate=rate, seed=seed, steps=steps)
+    srv = serve.Server(hbm_budget=200 << 30, chunk=25)
+    res = srv.run(reqs, realtime=True)
+    out = serve.summarize(res, srv.makespan)
+    srv.close()
+    out.update({"arrival_rate_per_s": rate, "steps_range": list(steps),
+                "fp64_requests": int(sum(1 for r in reqs if r.T == np.float64)),
+                "grids": {str(n): int(sum(1 for r in reqs if r.n == n)) for n in (32, 64, 128, 256)}})
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +178,12 @@ def main():
     ap.add_argument("--variants", type=str, default="",
                     help="fixed launch variants zinv,yinv,yfwd,zfwd,row instead of autotuning")
     ap.add_argument("--no-copy", action="store_true", help="skip the reference copy-bandwidth measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024^3 measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-steps", type=str, default="100,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (single-GPU build; "
@@ -145,6 +248,25 @@ def main():
         copy_gbs = round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)
         log(f"bench: reference copy kernel {copy_gbs} GB/s (read+write)")
 
+    large = None
+    if not args.no_large:
+        log("bench: 1024^3 Float32 MHD RK4")
+        del p
+        large = large_bench(M)
+        log(f"bench: 1024^3 {large}")
+
+    ensemble = None
+    if not args.no_ensemble:
+        log("bench: ensemble of 64 uneven runs vs sequential plain runs")
+        ensemble = ensemble_bench(M)
+        log(f"bench: ensemble {ensemble}")
+    served = None
+    if not args.no_serve:
+        log(f"bench: serving stream of {args.serve_requests} requests")
+        lo, hi = (int(v) for v in args.serve_steps.split(","))
+        served = serving_bench(args.serve_requests, args.serve_rate, (lo, hi))
+        log(f"bench: serving {served}")
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, literal structure)")
@@ -174,6 +296,9 @@ def main():
         "variants": {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
                      "zpass_fwd": chosen[3], "xrow_fused": chosen[4]},
         "cpu_baseline": cpu,
+        "n1024": large,
+        "ensemble": ensemble,
+        "serving": served,
         "zone_updates_per_s": round(value * n ** 3, 1),
         "E_kin": d["E_kin"], "E_mag": d["E_mag"],
     }
