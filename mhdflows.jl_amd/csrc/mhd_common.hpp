@@ -113,6 +113,8 @@ struct RunBuf {
   T* vars;            // nf real fields
   T* rscr;            // real scratch (>= 3 real fields), aliases the stage-input buffer
   T* partial;         // per-workgroup maxima (2 per row block), CFL input
+  cx<T>* plane;       // kx = 0 plane of the masked stage input, nf x nz x ny (zero off the kept set),
+                      // copied by the inverse z-pass for the spectral kernel's Hermitian projection
   RunClock* clk;
   const cx<T>* force; // forcing table (3 comps x (K+1) x (2K+1)^2) or null
   int force_K;

@@ -77,7 +77,8 @@ int64_t run_bytes(const Geom& g, int phys, int prec) {
   const int np = phys == PHYS_MHD ? 9 : 6;
   const int64_t rows = (int64_t)g.ny * g.nz;
   const int64_t blocks = prec == 0 ? row_blocks_max<float>(g.nx, rows) : row_blocks_max<double>(g.nx, rows);
-  return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s + 4096;
+  return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s +
+         (int64_t)nf * g.ny * g.nz * 2 * s + 4096;
 }
 
 template <class T>
@@ -178,6 +179,8 @@ class Engine : public EngineBase {
       b.vars = (T*)p;
       MHD_TRY(alloc(sizeof(T) * 2 * (size_t)row_blocks, &p));
       b.partial = (T*)p;
+      MHD_TRY(alloc(sizeof(cx<T>) * (size_t)nf * g.ny * g.nz, &p));
+      b.plane = (cx<T>*)p;
       MHD_TRY(alloc(sizeof(RunClock), &p));
       b.clk = (RunClock*)p;
       b.nu = rp[r].nu;
@@ -251,7 +254,7 @@ class Engine : public EngineBase {
   };
 
   int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
-              bool prune, bool hot = false) {
+              bool prune, bool hot = false, bool plane_out = false) {
     const int n = axis == 1 ? g.ny : g.nz;
     const int S = col_strip<T>(n, hot ? col_var[col_class(axis, dir)] : 0);
     ColArgs a{};
@@ -282,6 +285,7 @@ class Engine : public EngineBase {
     a.nruns = cur_n();
     a.nitems = (int64_t)cur_n() * nfields * a.nother * a.nstrips;
     a.gate = gate_;
+    a.plane_out = plane_out ? 1 : 0;
     const double cxb = sizeof(cx<T>);
     const double cols = (double)cur_n() * nfields * a.nother * a.ncols;
     const double bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
@@ -367,14 +371,18 @@ class Engine : public EngineBase {
     std::vector<int> best(5, 0);
     for (int cls = 0; cls < 5; ++cls) {
       float bms = 1e30f;
-      const int nvar = cls < 4 ? 3 : 2;
+      const int nvar = cls < 4 ? 3 : 4;
       std::string line = std::string(names[cls]) + ":";
       for (int v = 0; v < nvar; ++v) {
         if (cls < 4) {
           const int n = (cls == 0 || cls == 3) ? g.nz : g.ny;
           if (v > 0 && col_strip<T>(n, v) == col_strip<T>(n, 0)) continue;  // not a distinct shape
-        } else if (v > 0 && row_rows_per_block<T>(g.nx, v) == row_rows_per_block<T>(g.nx, 0)) {
-          continue;
+        } else {
+          // row variants: bit 0 plan, bit 1 padded LDS; skip a plan equal to a lower variant's
+          bool dup = false;
+          for (int u = 0; u < v; ++u)
+            if ((u >> 1) == (v >> 1) && row_rows_per_block<T>(g.nx, u) == row_rows_per_block<T>(g.nx, v)) dup = true;
+          if (dup) continue;
         }
         float ms = 0;
         MHD_TRY(time_variant(cls, v, reps, &ms));
@@ -412,7 +420,7 @@ class Engine : public EngineBase {
           col_var[c] = v5[c];
         }
       if (v5[4] >= 0) {
-        if (v5[4] > 1) return fail(E_INVALID, "row variant must be 0..1");
+        if (v5[4] > 3) return fail(E_INVALID, "row variant must be 0..3");
         row_var = v5[4];
       }
       if (gexec) {
@@ -444,7 +452,10 @@ class Engine : public EngineBase {
   // ---- one right-hand side evaluation ----------------------------------------
   // in_sel: stage input; prune: the stage input is dealiased; last: write vars + maxima.
   int rhs(int in_sel, bool prune, SpecArgs sp, bool last) {
-    MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune, true));
+    // a masked stage input has its kx = 0 plane copied aside by the z-pass, so the
+    // spectral launch covers every mode at once (kx = 0 mirrors read from the copy)
+    const bool fused_planes = prune && sp.mask_input && sp.self_sel < 0;
+    MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune, true, fused_planes));
     MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune, true));
     RowArgs ra = row_args(row_var);
     ra.in_sel = B_WORK;
@@ -463,6 +474,7 @@ class Engine : public EngineBase {
     sp.nitems = (int64_t)nr_ * g.nc;
     sp.prod_sel = B_WORK;
     sp.gate = gate_;
+    sp.planes = fused_planes ? 2 : 0;
     MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
     if (last) MHD_TRY(timed("reduce", 0, [&] {
       return launch_small<T>(SK_REDUCE, rbd, nr_, phys, used_blocks, sp.mode == SP_RHS ? 0 : 1, st);

@@ -22,6 +22,19 @@ struct alignas(2 * sizeof(T)) cx {
   T re, im;
 };
 
+// HBM buffers reached through pointers stored in memory (per-run descriptors) are
+// re-typed into the global address space: accesses then compile to global_* rather
+// than flat_* instructions, which would also count against the LDS wait counter and
+// serialise with LDS traffic.  (The host pass parses kernel bodies too; there the
+// qualifier has no meaning and the pointers stay plain.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class U> using gptr = __attribute__((address_space(1))) U*;
+#else
+template <class U> using gptr = U*;
+#endif
+template <class U> __device__ __forceinline__ gptr<U> gmem(U* p) { return (gptr<U>)p; }
+template <class U> __device__ __forceinline__ U ldg(gptr<const U> p) { return *p; }
+
 template <class T> __device__ __forceinline__ cx<T> mk(T a, T b) { return cx<T>{a, b}; }
 template <class T> __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
 template <class T> __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }

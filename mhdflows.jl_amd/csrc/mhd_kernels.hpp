@@ -42,6 +42,7 @@ struct ColArgs {
   int nruns;
   int64_t nitems;
   int gate;            // 1 inside a time step: skip runs whose clock reached its target
+  int plane_out;       // inverse z-pass of a masked stage input: copy the kx = 0 column into rb.plane
 };
 
 template <class T, int N, int S, int DIR>
@@ -76,8 +77,8 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const int col = strip * S + c;
   const bool col_ok = col < a.ncols;
   const int64_t base = (int64_t)o * ostride + col;
-  const cx<T>* __restrict__ src = rb.buf[a.src] + (int64_t)(a.src_f0 + f) * G.nc + base;
-  cx<T>* __restrict__ dst = rb.buf[a.dst] + (int64_t)(a.dst_f0 + f) * G.nc + base;
+  const gptr<const cx<T>> __restrict__ src = gmem<const cx<T>>(rb.buf[a.src]) + (int64_t)(a.src_f0 + f) * G.nc + base;
+  const gptr<cx<T>> __restrict__ dst = gmem(rb.buf[a.dst]) + (int64_t)(a.dst_f0 + f) * G.nc + base;
 
   cx<T> x[E];
 #pragma unroll
@@ -86,8 +87,15 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
     for (int r = 0; r < R0; ++r) {
       const int p = g + i * TH + r * (N / R0);
       const bool ok = col_ok && !(p >= a.mask_lo && p < a.mask_hi);
-      x[i * R0 + r] = ok ? src[p * estride] : cx<T>{T(0), T(0)};
+      x[i * R0 + r] = ok ? ldg(src + p * estride) : cx<T>{T(0), T(0)};
     }
+  }
+  if (a.plane_out && col == 0) {  // axis 2: o = l, positions p = m
+    const gptr<cx<T>> pl = gmem(rb.plane) + ((int64_t)f * G.nz) * G.ny + o;
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) pl[(int64_t)(g + i * TH + r * (N / R0)) * G.ny] = x[i * R0 + r];
   }
   fft_regs<T, N, E, DIR, Rad>(x, g, ldsf, tw);
   if (col_ok) {
@@ -174,23 +182,31 @@ __device__ __forceinline__ T product(int p, const T* v) {
 template <class T>
 __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
 
+// Row LDS index: with PAD one slot is skipped after every 8, which spreads the
+// strided exchange writes of the first Stockham passes over the LDS banks.
+template <bool PAD>
+__device__ __forceinline__ int rx(int pos) {
+  return PAD ? pos + (pos >> 3) : pos;
+}
+template <int M, bool PAD> constexpr int row_slots() { return PAD ? M + 1 + (M + 1) / 8 + 1 : M + 1; }
+
 // Forward real transform of the values z (one complex per pair of real samples,
 // in the pass-0 layout of Rfwd) and store the nkr half-spectrum entries.
-template <class T, int M, int E, int TX, class Rfwd, bool WS, class Lds>
+template <class T, int M, int E, int TX, class Rfwd, bool WS, bool PAD, class Lds>
 __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>* xrow,
                                               const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twh,
-                                              cx<T>* __restrict__ dst, bool row_ok) {
+                                              gptr<cx<T>> __restrict__ dst, bool row_ok) {
   constexpr int RfL = RLast<Rfwd>::value;
   fft_regs<T, M, E, -1, Rfwd, WS>(z, tr, ldsf, twM);
   xsync<WS>();
 #pragma unroll
   for (int i = 0; i < E / RfL; ++i)
 #pragma unroll
-    for (int r = 0; r < RfL; ++r) xrow[tr + i * TX + r * (M / RfL)] = z[i * RfL + r];
+    for (int r = 0; r < RfL; ++r) xrow[rx<PAD>(tr + i * TX + r * (M / RfL))] = z[i * RfL + r];
   xsync<WS>();
   for (int k = tr; k <= M; k += TX) {
-    const cx<T> Zk = xrow[k == M ? 0 : k];
-    const cx<T> Zm = conj(xrow[k == 0 ? 0 : M - k]);
+    const cx<T> Zk = xrow[rx<PAD>(k == M ? 0 : k)];
+    const cx<T> Zm = conj(xrow[rx<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> Ev = (Zk + Zm) * T(0.5);
     const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
     const cx<T> X = Ev + cmul(twh[k], Od);
@@ -203,7 +219,9 @@ __global__ void __launch_bounds__(128)
 xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
             const cx<T>* __restrict__ twh) {
   constexpr int M = NX / 2;
-  using RP = RowPlanV<T, M, VAR>;
+  // VAR bit 0 selects the plan (elements per thread), bit 1 the padded LDS index
+  using RP = RowPlanV<T, M, (VAR & 1)>;
+  constexpr bool PAD = (VAR & 2) != 0;
   constexpr int E = RP::E;
   constexpr int TX = M / E;
   static_assert(TX <= 128, "row transform too long for one workgroup");
@@ -220,7 +238,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   constexpr int NIN = RowKindInfo<KIND, NF>::NIN;
   constexpr int NPROD = RowKindInfo<KIND, NF>::NPROD;
 
-  __shared__ cx<T> xb[RPW][M + 1];
+  __shared__ cx<T> xb[RPW][row_slots<M, PAD>()];
   __shared__ T red[2][128];
 
   const int t = threadIdx.x;
@@ -236,9 +254,9 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   const bool row_ok = row < a.rows;
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
-  T* const rbase = a.real_sel ? rb.rscr : rb.vars;
+  const gptr<T> rbase = gmem(a.real_sel ? rb.rscr : rb.vars);
   cx<T>* xrow = xb[rw];
-  auto ldsf = [&](int pos) -> cx<T>& { return xrow[pos]; };
+  auto ldsf = [&](int pos) -> cx<T>& { return xrow[rx<PAD>(pos)]; };
 
   // v[f][e]: the real samples 2j and 2j+1 of field f at the thread's positions
   // j = tr + i*TX + r*(M/RL) (last inverse pass layout == first forward pass layout).
@@ -248,18 +266,19 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
     const T scale = T(G.inv_n);
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
-      const cx<T>* __restrict__ src = rb.buf[a.in_sel] + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
+      const gptr<const cx<T>> __restrict__ src =
+          gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
       xsync<WS>();
       for (int k = tr; k <= M; k += TX)
-        xrow[k] = (row_ok && k < a.in_ncols) ? src[k] : cx<T>{T(0), T(0)};
+        xrow[rx<PAD>(k)] = (row_ok && k < a.in_ncols) ? ldg(src + k) : cx<T>{T(0), T(0)};
       xsync<WS>();
 #pragma unroll
       for (int i = 0; i < E / R0; ++i) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
           const int p = tr + i * TX + r * (M / R0);
-          cx<T> Xk = xrow[p];
-          cx<T> Xm = xrow[M - p];
+          cx<T> Xk = xrow[rx<PAD>(p)];
+          cx<T> Xm = xrow[rx<PAD>(M - p)];
           if (p == 0) { Xk.im = T(0); Xm.im = T(0); }  // c2r ignores Im of the DC and Nyquist bins
           const cx<T> A = Xk + conj(Xm);
           const cx<T> B = Xk - conj(Xm);
@@ -273,13 +292,13 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
     if (a.write_vars && row_ok) {
 #pragma unroll
       for (int f = 0; f < NIN; ++f) {
-        T* __restrict__ vr = rbase + (int64_t)(a.real_out_f0 + f) * G.nr + row * NX;
+        const gptr<T> __restrict__ vr = rbase + (int64_t)(a.real_out_f0 + f) * G.nr + row * NX;
 #pragma unroll
         for (int i = 0; i < E / RL; ++i)
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
             const int j = tr + i * TX + r * (M / RL);
-            *reinterpret_cast<cx<T>*>(vr + 2 * j) = v[f][i * RL + r];
+            *(gptr<cx<T>>)(vr + 2 * j) = v[f][i * RL + r];
           }
       }
     }
@@ -306,21 +325,22 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
         __syncthreads();
       }
       if (t == 0) {
-        rb.partial[2 * blk] = red[0][0];
-        rb.partial[2 * blk + 1] = red[1][0];
+        const gptr<T> part = gmem(rb.partial);
+        part[2 * blk] = red[0][0];
+        part[2 * blk + 1] = red[1][0];
       }
     }
   } else {
     // real input rows (vars), loaded straight into the forward pass-0 layout
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
-      const T* __restrict__ vr = rbase + (int64_t)(a.in_f0 + f) * G.nr + row * NX;
+      const gptr<const T> __restrict__ vr = (gptr<const T>)rbase + (int64_t)(a.in_f0 + f) * G.nr + row * NX;
 #pragma unroll
       for (int i = 0; i < E / RL; ++i)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
           const int j = tr + i * TX + r * (M / RL);
-          v[f][i * RL + r] = row_ok ? *reinterpret_cast<const cx<T>*>(vr + 2 * j) : cx<T>{T(0), T(0)};
+          v[f][i * RL + r] = row_ok ? ldg((gptr<const cx<T>>)(vr + 2 * j)) : cx<T>{T(0), T(0)};
         }
     }
   }
@@ -348,10 +368,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 #pragma unroll
             for (int e = 0; e < E; ++e) z[e] = v[f][e];
       }
-      cx<T>* __restrict__ dst =
-          p < NPROD ? rb.buf[a.out_sel] + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
-                    : rb.buf[a.out2_sel] + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
-      row_r2c_store<T, M, E, TX, Rfwd, WS>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
+      const gptr<cx<T>> __restrict__ dst =
+          p < NPROD ? gmem(rb.buf[a.out_sel]) + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
+                    : gmem(rb.buf[a.out2_sel]) + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
+      row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
     }
   }
 }
@@ -378,7 +398,8 @@ struct SpecArgs {
   int self_f0;
   double ca, cb;      // LSRK54 stage coefficients A_i, B_i (rounded to T in the kernel)
   int64_t nitems;     // nruns * nc
-  int planes;         // 0: interior modes 1 <= kx < nx/2; 1: the kx = 0 and kx = nx/2 planes
+  int planes;         // 0: interior modes 1 <= kx < nx/2; 1: the kx = 0 and kx = nx/2 planes;
+                      // 2: every mode, kx = 0 mirrors read from rb.plane (masked stage input)
   int gate;           // 1 inside a time step: skip runs whose clock reached its target
 };
 
@@ -417,7 +438,7 @@ template <class T, int PHYS>
 __device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const SpecArgs& a,
                                        int64_t q, const ModeK<T>& k, const cx<T>* sH, cx<T>* R) {
   constexpr int NP = PHYS == PHYS_MHD ? 9 : 6;
-  const cx<T>* __restrict__ P = rb.buf[a.prod_sel];
+  const gptr<const cx<T>> __restrict__ P = gmem<const cx<T>>(rb.buf[a.prod_sel]);
   cx<T> p[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) p[i] = P[(int64_t)i * G.nc + q];
@@ -456,21 +477,28 @@ __device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const
   }
 }
 
-template <class T, int PHYS>
+template <class T, int PHYS, int PLANES>
 __global__ void __launch_bounds__(256)
 spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
   const Geom& G = a.g;
-  // grid: y = m, z = run.  Interior launch (a.planes == 0): x covers the modes
-  // 1 <= kx < nx/2 of every l.  Plane launch (a.planes == 1): x covers the kx = 0
+  // grid: y = m, z = run.  Interior launch (PLANES == 0): x covers the modes
+  // 1 <= kx < nx/2 of every l.  Plane launch (PLANES == 1): x covers the kx = 0
   // and kx = nx/2 planes, whose modes are handled in Hermitian pairs by the thread
   // of the lower index, so the pair's stage inputs are read before either is
   // overwritten.  Splitting keeps the interior waves free of the pair branch.
+  // Whole launch (PLANES == 2, masked stage input only): x covers every mode of the
+  // m slab; the mirror of a kx = 0 mode comes from the plane copy the inverse z-pass
+  // made, which this launch does not write, and the masked kx = nx/2 plane is zero.
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int run = blockIdx.z;
   const int m = blockIdx.y;
   int l, kx;
-  if (!a.planes) {
+  if constexpr (PLANES == 2) {
+    if (i >= (uint32_t)(G.ny * G.nkr)) return;
+    l = (int)fastdiv(i, G.fd_nkr_m, G.fd_nkr_l);
+    kx = (int)i - l * G.nkr;
+  } else if constexpr (PLANES == 0) {
     const int ni = G.nkr - 2;
     if (i >= (uint32_t)(G.ny * ni)) return;
     l = (int)fastdiv(i, G.fd_nin_m, G.fd_nin_l);
@@ -485,50 +513,71 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
 
-  const bool plane = a.planes != 0;
+  // Every per-mode array below is indexed by compile-time constants only (the mode
+  // loops have the fixed trip count NM and are unrolled), so it lives in registers;
+  // a runtime index would place it in scratch memory and add its traffic to HBM.
+  constexpr bool plane = PLANES == 1;
+  constexpr int NM = plane ? 2 : 1;
   int nmodes = 1;
-  int64_t qs[2] = {idx, idx};
-  int ls[2] = {l, l}, ms[2] = {m, m};
-  if (plane) {
+  int64_t qs[NM];
+  int ls[NM], ms[NM];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) { qs[j] = idx; ls[j] = l; ms[j] = m; }
+  if constexpr (plane) {
     const int lm = (G.ny - l) % G.ny, mm = (G.nz - m) % G.nz;
     const int64_t mir = ((int64_t)mm * G.ny + lm) * G.nkr + kx;
     if (mir < idx) return;
     if (mir != idx) { nmodes = 2; qs[1] = mir; ls[1] = lm; ms[1] = mm; }
   }
 
-  const cx<T>* __restrict__ in = rb.buf[a.in_sel];
+  const gptr<const cx<T>> __restrict__ in = gmem<const cx<T>>(rb.buf[a.in_sel]);
+  // Conditional reads are written as if-statements: a conditional operator over two
+  // lvalues selects between their addresses and would move `zero` and `s` to scratch.
   const cx<T> zero{T(0), T(0)};
-  cx<T> s[2][NF];     // masked stage input at the (up to) two modes
-  bool kept[2];
+  cx<T> s[NM][NF];     // masked stage input at the (up to) two modes
+  bool kept[NM];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < NM; ++j) {
     kept[j] = !a.mask_input || (kx < G.kx_keep && kept_y(G, ls[j]) && kept_z(G, ms[j]));
-    if (j < nmodes) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f)
-        s[j][f] = (kept[j] && a.self_sel < 0) ? in[(int64_t)f * G.nc + qs[j]] : zero;
+    for (int f = 0; f < NF; ++f) {
+      s[j][f] = zero;
+      if (j < nmodes && kept[j] && a.self_sel < 0) s[j][f] = in[(int64_t)f * G.nc + qs[j]];
     }
   }
   if (a.zero_aliased) {
-    cx<T>* __restrict__ io = rb.buf[a.in_sel];
-    for (int j = 0; j < nmodes; ++j)
-      if (!kept[j])
+    const gptr<cx<T>> __restrict__ io = gmem(rb.buf[a.in_sel]);
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+      if (j < nmodes && !kept[j])
 #pragma unroll
         for (int f = 0; f < NF; ++f) io[(int64_t)f * G.nc + qs[j]] = zero;
   }
 
-  for (int j = 0; j < nmodes; ++j) {
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    if (j >= nmodes) break;
     const int64_t q = qs[j];
     const ModeK<T> k = mode_k<T>(G, kx, ls[j], ms[j]);
     cx<T> sH[NF];
     if (a.self_sel >= 0) {
-      const cx<T>* __restrict__ sf = rb.buf[a.self_sel];
+      const gptr<const cx<T>> __restrict__ sf = gmem<const cx<T>>(rb.buf[a.self_sel]);
 #pragma unroll
       for (int f = 0; f < NF; ++f) sH[f] = sf[(int64_t)(a.self_f0 + f) * G.nc + q];
     } else if (plane) {
-      // R2C(C2R(s)) = Hermitian projection on this plane
+      // R2C(C2R(s)) = Hermitian projection on this plane; a self-conjugate mode is its own mirror
 #pragma unroll
-      for (int f = 0; f < NF; ++f) sH[f] = (s[j][f] + conj(s[nmodes - 1 - j][f])) * T(0.5);
+      for (int f = 0; f < NF; ++f) {
+        cx<T> mir = s[j][f];
+        if (nmodes == 2) mir = s[NM - 1 - j][f];
+        sH[f] = (s[j][f] + conj(mir)) * T(0.5);
+      }
+    } else if (PLANES == 2 && kx == 0) {
+      const int lm = l == 0 ? 0 : G.ny - l, mm = m == 0 ? 0 : G.nz - m;
+      const gptr<const cx<T>> pl = gmem<const cx<T>>(rb.plane) + (int64_t)mm * G.ny + lm;
+      const int64_t pstride = (int64_t)G.ny * G.nz;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sH[f] = (s[j][f] + conj(ldg(pl + f * pstride))) * T(0.5);
     } else {
 #pragma unroll
       for (int f = 0; f < NF; ++f) sH[f] = s[j][f];
@@ -547,14 +596,16 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
     }
 
     if (a.mode == SP_RHS) {
-      cx<T>* __restrict__ out = rb.buf[a.out_sel];
-      for (int f = a.f_lo; f < a.f_hi; ++f) out[(int64_t)(a.out_f0 + f - a.f_lo) * G.nc + q] = R[f];
+      const gptr<cx<T>> __restrict__ out = gmem(rb.buf[a.out_sel]);
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        if (f >= a.f_lo && f < a.f_hi) out[(int64_t)(a.out_f0 + f - a.f_lo) * G.nc + q] = R[f];
       continue;
     }
     const T dt = T(rb.clk->dt);
-    cx<T>* __restrict__ sol = rb.buf[B_SOL];
-    cx<T>* __restrict__ acc = rb.buf[B_ACC];
-    cx<T>* __restrict__ stin = rb.buf[B_STIN];
+    const gptr<cx<T>> __restrict__ sol = gmem(rb.buf[B_SOL]);
+    const gptr<cx<T>> __restrict__ acc = gmem(rb.buf[B_ACC]);
+    const gptr<cx<T>> __restrict__ stin = gmem(rb.buf[B_STIN]);
     if (a.mode == SP_RK4) {
       // RK4substeps! / RK4update! (DyeModule.jl:62-91): acc holds the running sum
       // RHS1/6 + RHS2/3 + RHS3/3 in the reference's left-to-right order.
@@ -562,7 +613,11 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int64_t o = (int64_t)f * G.nc + q;
-        const cx<T> base = kept[j] ? (a.stage == 0 ? s[j][f] : sol[o]) : zero;
+        cx<T> base = zero;
+        if (kept[j]) {
+          if (a.stage == 0) base = s[j][f];
+          else base = sol[o];
+        }
         if (a.stage == 0) {
           acc[o] = cx<T>{R[f].re / T(6), R[f].im / T(6)};
           if (kept[j]) stin[o] = base + R[f] * half;
@@ -582,7 +637,8 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int64_t o = (int64_t)f * G.nc + q;
-        const cx<T> Sold = a.stage == 0 ? zero : acc[o];
+        cx<T> Sold = zero;
+        if (a.stage != 0) Sold = acc[o];
         const cx<T> Snew = Sold * A + R[f] * dt;
         acc[o] = Snew;
         sol[o] = s[j][f] + Snew * B;

@@ -28,7 +28,7 @@ template <class T> inline int col_threads(int n) {
 }
 template <class T> inline int row_rows_per_block(int nx, int var = 0) {
   const int m = nx / 2;
-  const int tx = m / row_plan_E(m, sizeof(T) == 8, var);
+  const int tx = m / row_plan_E(m, sizeof(T) == 8, var & 1);
   return 128 / tx;
 }
 // most row blocks any variant needs (sizes the per-block partials)

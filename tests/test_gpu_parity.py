@@ -290,7 +290,8 @@ def test_rhs_parity_fp32_three_factor(n):
     assert rel_l2(N_gpu, N_ref) < 2e-5
 
 
-@pytest.mark.parametrize("var", [[1, 1, 1, 1, 1], [2, 2, 2, 2, 0], [0, 2, 1, 0, 1]])
+@pytest.mark.parametrize("var", [[1, 1, 1, 1, 1], [2, 2, 2, 2, 0], [0, 2, 1, 0, 1], [0, 0, 0, 0, 2],
+                                 [1, 2, 0, 1, 3]])
 @pytest.mark.parametrize("T,n,tol", [(np.float32, 64, 2e-5), (np.float64, 32, 1e-12)])
 def test_variants_parity(var, T, n, tol):
     say("variants", var, T.__name__, n)
