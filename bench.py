@@ -23,6 +23,24 @@ for _p in (ROOT, os.path.join(ROOT, "mhdflows.jl_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+COUNTER_PROFILE = os.path.join("profiles", "mhd256_fp32_rk4_launches.md")
+
+
+def counter_traffic(launch, variants):
+    """HBM bytes per launch of `launch` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration recorded there),
+    or None when the summary lacks the launch or was taken with another variant of it."""
+    try:
+        txt = open(os.path.join(ROOT, COUNTER_PROFILE)).read()
+        blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return None, None
+    rec = blob.get("launches", {}).get(launch)
+    if rec is None or blob.get("variants", {}).get(launch) != variants.get(launch):
+        return None, None
+    src = (f"{COUNTER_PROFILE}: FETCH_SIZE x{blob['calibration']['fetch_factor']} + WRITE_SIZE "
+           f"x{blob['calibration']['write_factor']} per dispatch of {rec['kernel']} (copy-kernel calibrated)")
+    return rec["traffic_bytes"], src
 
 
 def log(*a):
@@ -232,9 +250,13 @@ def main():
     avg_ms = dom["ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    chosen_named = {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
+                    "zpass_fwd": chosen[3], "xrow_fused": chosen[4]}
+    traffic, traffic_src = counter_traffic(dom["name"], chosen_named)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
-                "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
+                "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch,
+                "traffic_source": traffic_src}
     step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
     kernels = {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4),
                            "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
@@ -293,8 +315,7 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "copy_kernel_GBps": copy_gbs,
-        "variants": {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
-                     "zpass_fwd": chosen[3], "xrow_fused": chosen[4]},
+        "variants": chosen_named,
         "cpu_baseline": cpu,
         "n1024": large,
         "ensemble": ensemble,
