@@ -204,13 +204,16 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
 #pragma unroll
     for (int r = 0; r < RfL; ++r) xrow[rx<PAD>(tr + i * TX + r * (M / RfL))] = z[i * RfL + r];
   xsync<WS>();
-  for (int k = tr; k <= M; k += TX) {
-    const cx<T> Zk = xrow[rx<PAD>(k == M ? 0 : k)];
+  // entries k < M; the Nyquist entry M shares Z_0 with k = 0 and is finished there
+  for (int k = tr; k < M; k += TX) {
+    const cx<T> Zk = xrow[rx<PAD>(k)];
     const cx<T> Zm = conj(xrow[rx<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> Ev = (Zk + Zm) * T(0.5);
     const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
-    const cx<T> X = Ev + cmul(twh[k], Od);
-    if (row_ok) dst[k] = X;
+    if (row_ok) {
+      dst[k] = Ev + cmul(twh[k], Od);
+      if (k == 0) dst[M] = Ev + cmul(twh[M], Od);
+    }
   }
 }
 
@@ -240,6 +243,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 
   __shared__ cx<T> xb[RPW][row_slots<M, PAD>()];
   __shared__ T red[2][128];
+  __shared__ cx<T> stw[2 * M + 1];  // twiddles: M of the length-M transform, then M+1 of the real pack
 
   const int t = threadIdx.x;
   const int rw = t / TX;
@@ -254,6 +258,12 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   const bool row_ok = row < a.rows;
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
+  // The passes read their twiddles from LDS: a table load from the cache hierarchy would
+  // sit in every pass's dependency chain.
+  for (int i = t; i < 2 * M + 1; i += 128) stw[i] = i < M ? twM[i] : twh[i - M];
+  __syncthreads();
+  const cx<T>* const sM = stw;
+  const cx<T>* const sh = stw + M;
   const gptr<T> rbase = gmem(a.real_sel ? rb.rscr : rb.vars);
   cx<T>* xrow = xb[rw];
   auto ldsf = [&](int pos) -> cx<T>& { return xrow[rx<PAD>(pos)]; };
@@ -269,8 +279,9 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       const gptr<const cx<T>> __restrict__ src =
           gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
       xsync<WS>();
-      for (int k = tr; k <= M; k += TX)
+      for (int k = tr; k < M; k += TX)
         xrow[rx<PAD>(k)] = (row_ok && k < a.in_ncols) ? ldg(src + k) : cx<T>{T(0), T(0)};
+      if (tr == 0) xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldg(src + M) : cx<T>{T(0), T(0)};
       xsync<WS>();
 #pragma unroll
       for (int i = 0; i < E / R0; ++i) {
@@ -282,10 +293,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
           if (p == 0) { Xk.im = T(0); Xm.im = T(0); }  // c2r ignores Im of the DC and Nyquist bins
           const cx<T> A = Xk + conj(Xm);
           const cx<T> B = Xk - conj(Xm);
-          v[f][i * R0 + r] = A + mul_si<1>(cmulc(B, twh[p]));
+          v[f][i * R0 + r] = A + mul_si<1>(cmulc(B, sh[p]));
         }
       }
-      fft_regs<T, M, E, 1, Rinv, WS>(v[f], tr, ldsf, twM);
+      fft_regs<T, M, E, 1, Rinv, WS>(v[f], tr, ldsf, sM);
 #pragma unroll
       for (int e = 0; e < E; ++e) v[f][e] = v[f][e] * scale;
     }
@@ -371,7 +382,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       const gptr<cx<T>> __restrict__ dst =
           p < NPROD ? gmem(rb.buf[a.out_sel]) + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
                     : gmem(rb.buf[a.out2_sel]) + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
-      row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, twM, twh, dst, row_ok);
+      row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, sM, sh, dst, row_ok);
     }
   }
 }
