@@ -167,6 +167,91 @@ def ensemble_bench(M, steps=20):
             "sequential_run_steps_per_s": round(seq_rate, 1), "speedup": round(ens_rate / seq_rate, 2)}
 
 
+def ensemble64_bench(M, nruns=64, steps=20, seq_runs=16):
+    """64 runs of 64^3 (Float32, LSRK54, CFL 0.3, k0 ~ U[2,6], nu = eta log-uniform in
+    [1e-3, 1e-2]) stepped as one ensemble, against the same runs stepped one after another
+    (a) each on its own captured step with the device-side CFL and (b) the reference's
+    loop, getCFL! read back on the host before every stepforward!.  The sequential rates
+    come from the first `seq_runs` runs (same size, so the same cost per run)."""
+    import numpy as np
+    from mhdflows_amd import ensemble as E
+    from mhdflows_amd import ic
+    from mhdflows_amd.devmem import hip
+    rng = np.random.Generator(np.random.PCG64(11))
+    specs = [(float(rng.uniform(2, 6)), float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-2)))),
+              int(rng.integers(1 << 30))) for _ in range(nruns)]
+    modes = [ic.solenoidal_modes(s, k0=k0, kmax=16.0).half_entries(64) for (k0, nu, s) in specs]
+    ens = E.Ensemble([E.make_config(64, nu=nu, eta=nu, T=np.float32, stepper="LSRK54") for (k0, nu, s) in specs])
+    for r, (idx, vals) in enumerate(modes):
+        ens.set_ic_modes(r, idx, vals)
+        ens.set_cfl(r, 0.3)
+    ens.autotune(3)
+    ens.step(2)
+    ens.sync()
+    ms = ens.time_steps(steps)
+    ens_rate = nruns * steps / (ms * 1e-3)
+    ens.close()
+    t_graph = t_host = 0.0
+    for (k0, nu, s), (idx, vals) in list(zip(specs, modes))[:seq_runs]:
+        p = M.Problem("MI355X", nx=64, ν=nu, η=nu, B_field=True, T=np.float32, stepper="LSRK54")
+        M.SetUpProblemIC_modes(p, idx, vals)
+        M.TimeIntegrator(p, math.inf, 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        t_graph += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            M.getCFL(p, Coef=0.3)
+            M.stepforward(p, 1)
+        hip().hipDeviceSynchronize()
+        t_host += time.perf_counter() - t0
+        del p
+    seq_graph = seq_runs * steps / t_graph
+    seq_host = seq_runs * steps / t_host
+    return {"workload": "64 runs of 64^3 MHD, Float32 LSRK54, CFL 0.3", "steps": steps,
+            "ensemble_run_steps_per_s": round(ens_rate, 1),
+            "sequential_captured_run_steps_per_s": round(seq_graph, 1),
+            "sequential_host_cfl_run_steps_per_s": round(seq_host, 1),
+            "speedup_vs_captured": round(ens_rate / seq_graph, 2),
+            "speedup_vs_host_cfl": round(ens_rate / seq_host, 2)}
+
+
+def served64_bench(M, nreq=64, steps=(100, 300), seed=5, seq_runs=8):
+    """64 requests of 64^3 Float32 MHD (RK4, CFL 0.3, nu = eta log-uniform in [1e-4, 1e-2],
+    100-300 steps) arriving together and served by continuous batching in one slot pool,
+    against the same requests solved one after another on the single-run captured path
+    (rate from the first `seq_runs` requests)."""
+    import numpy as np
+    from mhdflows_amd import serve
+    from mhdflows_amd.devmem import hip
+    reqs = serve.poisson_stream(nreq, rate=1e6, seed=seed, steps=steps, sizes=(64,), weights=(1.0,),
+                                fp64_frac=0.0)
+    srv = serve.Server(hbm_budget=200 << 30, slots={64: nreq}, chunk=25)
+    res = srv.run(reqs, realtime=False)
+    out = serve.summarize(res, srv.makespan)
+    srv.close()
+    t = 0.0
+    nsteps = 0
+    for r in reqs[:seq_runs]:
+        p = M.Problem("MI355X", nx=64, ν=r.nu, η=r.eta, B_field=True, T=np.float32, stepper="RK4")
+        idx, vals = serve.initial_modes(r)
+        M.SetUpProblemIC_modes(p, idx, vals)
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        M.TimeIntegrator(p, math.inf, r.steps - 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        t += time.perf_counter() - t0
+        nsteps += r.steps
+        del p
+    seq = nsteps / t
+    return {"workload": f"{nreq} requests of 64^3 MHD Float32 RK4, {steps[0]}-{steps[1]} steps, arriving together",
+            "served_steps_per_s": round(out["aggregate_steps_per_s"], 1),
+            "p50_latency_s": round(out["p50_latency_s"], 4), "p99_latency_s": round(out["p99_latency_s"], 4),
+            "sequential_steps_per_s": round(seq, 1), "speedup": round(out["aggregate_steps_per_s"] / seq, 2)}
+
+
 def serving_bench(nreq, rate, steps, seed=7):
     """Poisson stream of MHD solve requests (grid 32/64/128/256 with weights .4/.3/.2/.1,
     Float32/Float64 90/10, nu = eta log-uniform in [1e-4, 1e-2], seeded k^4 exp(-2(k/4)^2)
@@ -282,12 +367,18 @@ def main():
         log("bench: ensemble of 64 uneven runs vs sequential plain runs")
         ensemble = ensemble_bench(M)
         log(f"bench: ensemble {ensemble}")
+        log("bench: 64 runs of 64^3 as one ensemble vs sequential runs")
+        ensemble["n64"] = ensemble64_bench(M)
+        log(f"bench: ensemble 64^3 {ensemble['n64']}")
     served = None
     if not args.no_serve:
         log(f"bench: serving stream of {args.serve_requests} requests")
         lo, hi = (int(v) for v in args.serve_steps.split(","))
         served = serving_bench(args.serve_requests, args.serve_rate, (lo, hi))
         log(f"bench: serving {served}")
+        log("bench: 64 requests of 64^3 served together vs sequential solves")
+        served["n64"] = served64_bench(M)
+        log(f"bench: served 64^3 {served['n64']}")
 
     cpu = None
     if not args.no_cpu:
