@@ -176,7 +176,7 @@ int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out);
 /* Run nsteps steps with an event pair around every kernel launch and report per
  * kernel class launches, time and algorithmic bytes (at most max entries). */
 int32_t mhd_profile_steps(mhd_handle* h, int32_t nsteps, mhd_ktime* out, int32_t max, int32_t* n_out);
-/* Reference streaming copy of `bytes` bytes (16 B per lane), timed over reps launches:
+/* Reference streaming copy of `bytes` bytes (one 16 B element per lane), timed over reps launches:
  * average ms per launch.  Used to calibrate HBM byte counters and as the practical
  * bandwidth ceiling next to the 8 TB/s peak. */
 int32_t mhd_bench_copy(uint64_t bytes, int32_t reps, double* ms_out);

@@ -35,6 +35,29 @@ template <class U> using gptr = U*;
 template <class U> __device__ __forceinline__ gptr<U> gmem(U* p) { return (gptr<U>)p; }
 template <class U> __device__ __forceinline__ U ldg(gptr<const U> p) { return *p; }
 
+// Loads and stores of the hot kernels' streams (each element read or written once per
+// launch).  NT = true adds the non-temporal hint.  Measured on MI355X: it speeds up the
+// pointwise spectral launch (loads and stores) and the row loads, and slows the column
+// passes; non-temporal row stores (1032 B rows, not line aligned) add partial-line writes.
+template <bool NT, class T> __device__ __forceinline__ cx<T> ldst(gptr<const cx<T>> p) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load((gptr<const v2>)p);
+    return cx<T>{v.x, v.y};
+  } else {
+    return *p;
+  }
+}
+template <bool NT, class T> __device__ __forceinline__ void stst(gptr<cx<T>> p, cx<T> x) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = {x.re, x.im};
+    __builtin_nontemporal_store(v, (gptr<v2>)p);
+  } else {
+    *p = x;
+  }
+}
+
 template <class T> __device__ __forceinline__ cx<T> mk(T a, T b) { return cx<T>{a, b}; }
 template <class T> __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
 template <class T> __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }

@@ -87,7 +87,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
     for (int r = 0; r < R0; ++r) {
       const int p = g + i * TH + r * (N / R0);
       const bool ok = col_ok && !(p >= a.mask_lo && p < a.mask_hi);
-      x[i * R0 + r] = ok ? ldg(src + p * estride) : cx<T>{T(0), T(0)};
+      x[i * R0 + r] = ok ? ldst<false, T>(src + p * estride) : cx<T>{T(0), T(0)};
     }
   }
   if (a.plane_out && col == 0) {  // axis 2: o = l, positions p = m
@@ -104,7 +104,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
         const int p = g + i * TH + r * (N / RL);
-        dst[p * estride] = x[i * RL + r];
+        stst<false, T>(dst + p * estride, x[i * RL + r]);
       }
     }
   }
@@ -211,8 +211,8 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
     const cx<T> Ev = (Zk + Zm) * T(0.5);
     const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
     if (row_ok) {
-      dst[k] = Ev + cmul(twh[k], Od);
-      if (k == 0) dst[M] = Ev + cmul(twh[M], Od);
+      stst<false, T>(dst + k, Ev + cmul(twh[k], Od));
+      if (k == 0) stst<false, T>(dst + M, Ev + cmul(twh[M], Od));
     }
   }
 }
@@ -280,8 +280,8 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
           gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
       xsync<WS>();
       for (int k = tr; k < M; k += TX)
-        xrow[rx<PAD>(k)] = (row_ok && k < a.in_ncols) ? ldg(src + k) : cx<T>{T(0), T(0)};
-      if (tr == 0) xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldg(src + M) : cx<T>{T(0), T(0)};
+        xrow[rx<PAD>(k)] = (row_ok && k < a.in_ncols) ? ldst<true, T>(src + k) : cx<T>{T(0), T(0)};
+      if (tr == 0) xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldst<true, T>(src + M) : cx<T>{T(0), T(0)};
       xsync<WS>();
 #pragma unroll
       for (int i = 0; i < E / R0; ++i) {
@@ -452,7 +452,7 @@ __device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const
   const gptr<const cx<T>> __restrict__ P = gmem<const cx<T>>(rb.buf[a.prod_sel]);
   cx<T> p[NP];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) p[i] = P[(int64_t)i * G.nc + q];
+  for (int i = 0; i < NP; ++i) p[i] = ldst<true, T>(P + (int64_t)i * G.nc + q);
   const cx<T> zero{T(0), T(0)};
   if constexpr (PHYS == PHYS_MHD) {
     // D_j = sum_i i k_i T_ij ; N_u = D - k (k.D)/k^2  (UᵢUpdate!, MHDSolver.jl:63-83)
@@ -553,7 +553,7 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       s[j][f] = zero;
-      if (j < nmodes && kept[j] && a.self_sel < 0) s[j][f] = in[(int64_t)f * G.nc + qs[j]];
+      if (j < nmodes && kept[j] && a.self_sel < 0) s[j][f] = ldst<true, T>(in + (int64_t)f * G.nc + qs[j]);
     }
   }
   if (a.zero_aliased) {
@@ -627,19 +627,19 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
         cx<T> base = zero;
         if (kept[j]) {
           if (a.stage == 0) base = s[j][f];
-          else base = sol[o];
+          else base = ldst<true, T>(sol + o);
         }
         if (a.stage == 0) {
-          acc[o] = cx<T>{R[f].re / T(6), R[f].im / T(6)};
-          if (kept[j]) stin[o] = base + R[f] * half;
+          stst<true, T>(acc + o, cx<T>{R[f].re / T(6), R[f].im / T(6)});
+          if (kept[j]) stst<true, T>(stin + o, base + R[f] * half);
         } else if (a.stage == 1 || a.stage == 2) {
-          const cx<T> ac = acc[o];
-          acc[o] = cx<T>{ac.re + R[f].re / T(3), ac.im + R[f].im / T(3)};
-          if (kept[j]) stin[o] = base + R[f] * (a.stage == 1 ? half : dt);
+          const cx<T> ac = ldst<true, T>(acc + o);
+          stst<true, T>(acc + o, cx<T>{ac.re + R[f].re / T(3), ac.im + R[f].im / T(3)});
+          if (kept[j]) stst<true, T>(stin + o, base + R[f] * (a.stage == 1 ? half : dt));
         } else {
-          const cx<T> ac = acc[o];
+          const cx<T> ac = ldst<true, T>(acc + o);
           const cx<T> tot{ac.re + R[f].re / T(6), ac.im + R[f].im / T(6)};
-          sol[o] = base + tot * dt;
+          stst<true, T>(sol + o, base + tot * dt);
         }
       }
     } else {
@@ -649,10 +649,10 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
       for (int f = 0; f < NF; ++f) {
         const int64_t o = (int64_t)f * G.nc + q;
         cx<T> Sold = zero;
-        if (a.stage != 0) Sold = acc[o];
+        if (a.stage != 0) Sold = ldst<true, T>(acc + o);
         const cx<T> Snew = Sold * A + R[f] * dt;
-        acc[o] = Snew;
-        sol[o] = s[j][f] + Snew * B;
+        stst<true, T>(acc + o, Snew);
+        stst<true, T>(sol + o, s[j][f] + Snew * B);
       }
     }
   }
