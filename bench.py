@@ -8,6 +8,9 @@ u_rms = b_rms = 1, cross-helicity 0.  Inputs are resident in HBM before timing.
 One JSON line on stdout.  `roofline` is measured with HIP events around every
 launch of a separate profiled pass; `cpu_baseline` times the oracle's literal
 (reference-structured, 36 FFTs per right-hand side) path on the host cores.
+`value` uses the library's default dealiased steps (every step of the integration but
+the last updates the kept modes only; the final state is the same bit for bit);
+`every_step_full` repeats the timing with every step computed in full.
 """
 
 import argparse
@@ -24,6 +27,20 @@ for _p in (ROOT, os.path.join(ROOT, "mhdflows.jl_amd")):
 
 HBM_PEAK_GBS = 8000.0
 COUNTER_PROFILE = os.path.join("profiles", "mhd256_fp32_rk4_launches.md")
+
+
+def counter_step_traffic(variants):
+    """HBM bytes of one right-hand side's six launches x 4 (an RK4 step) from the committed
+    counter summary, or (None, None) when it was taken with other launch variants."""
+    try:
+        txt = open(os.path.join(ROOT, COUNTER_PROFILE)).read()
+        blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return None, None
+    if blob.get("variants") != variants:
+        return None, None
+    tot = 4 * sum(rec["traffic_bytes"] for rec in blob["launches"].values())
+    return round(tot), f"{COUNTER_PROFILE}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes, same mix of steps"
 
 
 def counter_traffic(launch, variants):
@@ -69,6 +86,12 @@ def profile(M, p, nsteps):
     return out
 
 
+def executed_bytes_per_step(prof, nsteps):
+    """Algorithmic bytes of the launches a profiled run executed, per step (dealiased steps
+    move fewer bytes than the last, full step of a call)."""
+    return sum(k["bytes"] for k in prof) / nsteps
+
+
 def cpu_baseline(fields, n, steps=1):
     import numpy as np
     from oracle import OracleProblem
@@ -110,14 +133,28 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
     M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
     hip().hipDeviceSynchronize()
     wall = time.perf_counter() - t0
-    step_bytes = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
     ms = wall * 1e3 / steps
-    gbs = step_bytes / (ms * 1e-3) / 1e9
+    full_model = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
+    exec_bytes = executed_bytes_per_step(profile(M, p, steps), steps)
+    gbs = full_model / (ms * 1e-3) / 1e9
     d = M.Diagnostics(p)
+    M.set_step_pruning(p, False)
+    hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
+    hip().hipDeviceSynchronize()
+    wall_full = time.perf_counter() - t0
+    M.set_step_pruning(p, True)
+    gbs_full = full_model / (wall_full / steps) / 1e9
     out = {"grid": n, "steps": steps, "value": round(steps / wall, 4), "unit": "steps/s",
            "ms_per_step": round(ms, 2), "variants": M.variants(p),
-           "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(gbs, 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
+           "step_roofline": {"model_bytes_per_step": full_model, "achieved": round(gbs, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "executed_alg_bytes_per_step": round(exec_bytes),
+                             "executed_alg_GBps": round(exec_bytes / (ms * 1e-3) / 1e9, 1)},
+           "every_step_full": {"value": round(steps / wall_full, 4), "ms_per_step": round(wall_full * 1e3 / steps, 2),
+                               "model_bytes_per_step": full_model, "achieved": round(gbs_full, 1),
+                               "frac": round(gbs_full / HBM_PEAK_GBS, 4)},
            "finite": not d["has_nan"]}
     del p
     return out
@@ -275,7 +312,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps profiled per launch (default: --steps, the timed run's mix of steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--variants", type=str, default="",
@@ -328,8 +366,18 @@ def main():
     if d["has_nan"]:
         raise SystemExit("NaN in the benchmark run")
 
-    log(f"bench: profiling {args.profile_steps} steps with per-launch HIP events")
-    prof = profile(M, p, args.profile_steps)
+    nprof = args.profile_steps or args.steps
+    log(f"bench: profiling {nprof} steps with per-launch HIP events")
+    prof = profile(M, p, nprof)
+    exec_bytes = executed_bytes_per_step(prof, nprof)
+    M.set_step_pruning(p, False)
+    hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    M.TimeIntegrator(p, math.inf, args.steps - 1, CFL_Coef=0.3)
+    hip().hipDeviceSynchronize()
+    wall_full = time.perf_counter() - t0
+    M.set_step_pruning(p, True)
+    log(f"bench: every step full: {args.steps / wall_full:.2f} steps/s")
     kern = [k for k in prof if k["bytes"] > 0]
     dom = max(kern, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
@@ -343,6 +391,8 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch,
                 "traffic_source": traffic_src}
     step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
+    full_gbs = step_bytes / (wall_full / args.steps) / 1e9
+    dram = counter_step_traffic(chosen_named)
     kernels = {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4),
                            "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
                            if k["bytes"] > 0 else None} for k in prof}
@@ -403,7 +453,16 @@ def main():
                    "grid": n, "precision": "Float32", "stepper": "RK4", "parallelism": "single GPU"},
         "roofline": roofline,
         "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                          "bytes_basis": "fixed 3-pass model of the step (MHD RK4: 438 V_c + 120 V_r), "
+                                         "independent of the implementation",
+                          "executed_alg_bytes_per_step": round(exec_bytes),
+                          "executed_alg_GBps": round(exec_bytes / (ms_step * 1e-3) / 1e9, 1),
+                          "dram_bytes_per_step": dram[0], "dram_source": dram[1]},
+        "every_step_full": {"value": round(args.steps / wall_full, 3),
+                            "ms_per_step": round(wall_full * 1e3 / args.steps, 4),
+                            "model_bytes_per_step": step_bytes, "achieved": round(full_gbs, 1),
+                            "frac": round(full_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "copy_kernel_GBps": copy_gbs,
         "variants": chosen_named,

@@ -150,6 +150,14 @@ int32_t mhd_integrate(mhd_handle* h, double t0, int64_t N0, double usr_dt, doubl
 /* Capture one step into a HIP graph and replay it for subsequent mhd_step calls
  * (enable = 1) or launch kernels one by one (enable = 0). */
 int32_t mhd_use_graph(mhd_handle* h, int32_t enable);
+/* Dealiased steps (default enable = 1): a stage input is dealiased before use, so the
+ * right-hand side at aliased modes never reaches a kept mode; it only sets the aliased
+ * modes of sol at the end of a step, which the next step discards.  Every step of an
+ * mhd_step / mhd_integrate (t0 = Inf) call except the last therefore transforms, stores
+ * and updates the kept modes only; the last step is computed in full, so the state seen
+ * after the call equals the one every-step-full stepping gives (enable = 0).  Has no
+ * counterpart in the reference, which computes every step in full. */
+int32_t mhd_set_step_pruning(mhd_handle* h, int32_t enable);
 
 /* ---- diagnostics and forcing -------------------------------------------------- */
 /* FP64 reductions over the current vars (the reference's diagnostics read vars). */
@@ -168,7 +176,8 @@ int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp);
 int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len);
 /* Set (v5 != NULL; negative entries keep the current value) and/or read (out5 != NULL)
  * the launch variants {zpass_inv, ypass_inv, ypass_fwd, zpass_fwd: strip width
- * 0 = 128 B, 1 = 64 B, 2 = 256 B; xrow_fused: plan 0/1}.  Results do not depend on
+ * 0 = 128 B, 1 = 64 B, 2 = 256 B, 3 = 512 B of kx per row; xrow_fused: bit 0 plan, bit 1 padded
+ * LDS (0..3)}.  Results do not depend on
  * the variant beyond floating-point rounding of the transform order. */
 int32_t mhd_set_variants(mhd_handle* h, const int32_t* v5, int32_t* out5);
 /* Time nsteps steps with HIP events on the handle's stream (whole-step wall time). */
@@ -211,6 +220,9 @@ int32_t mhd_batch_start_run(mhd_batch* b, int32_t run, double dt, double cfl_coe
 /* Enqueue nsteps steps of every run without waiting; query reports 1 while work is
  * pending; sync waits for it. */
 int32_t mhd_batch_launch(mhd_batch* b, int32_t nsteps);
+/* mhd_set_step_pruning for every run of the batch (per run, the last step of a launch or
+ * the step reaching its target is the full one). */
+int32_t mhd_batch_set_step_pruning(mhd_batch* b, int32_t enable);
 int32_t mhd_batch_query(mhd_batch* b, int32_t* busy);
 int32_t mhd_batch_sync(mhd_batch* b);
 int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps);

@@ -272,6 +272,10 @@ int32_t mhd_bench_copy(uint64_t bytes, int32_t reps, double* ms_out) {
   hipFree(b);
   return 0;
 }
+int32_t mhd_set_step_pruning(mhd_handle* h, int32_t enable) {
+  if (!h) return fail(E_INVALID, "handle is NULL");
+  return h->e->set_pruning(enable);
+}
 int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out) {
   MHD_TRY(check_handle(h));
   if (!ms_out) return fail(E_INVALID, "ms_out is NULL");
@@ -395,6 +399,11 @@ int32_t mhd_batch_start_run(mhd_batch* b, int32_t run, double dt, double cfl_coe
   BATCH_RUN(b, run);
   if (target_steps < 0) return fail(E_INVALID, "target_steps must be >= 0");
   return G_.e->start_run(L_, dt, cfl_coef, target_steps);
+}
+int32_t mhd_batch_set_step_pruning(mhd_batch* b, int32_t enable) {
+  if (!b) return fail(E_INVALID, "batch is NULL");
+  for (auto& g : b->groups) MHD_TRY(g.e->set_pruning(enable));
+  return 0;
 }
 int32_t mhd_batch_launch(mhd_batch* b, int32_t nsteps) {
   if (!b) return fail(E_INVALID, "batch is NULL");

@@ -97,6 +97,7 @@ struct RunClock {
   int32_t use_cfl;   // 1: dt from getCFL! before each step
   int32_t nan_flag;  // set when the maxima are NaN
   int64_t target;    // step launches skip this run once step >= target (pool slots)
+  int64_t prune_until;  // steps ending before this step count skip the aliased modes (see step_pruned)
 };
 
 // Step launches of a run whose clock reached its target do nothing (finished or
@@ -104,6 +105,15 @@ struct RunClock {
 template <class RB>
 __device__ __forceinline__ bool gated_off(int gate, const RB& rb) {
   return gate && rb.clk->step >= rb.clk->target;
+}
+
+// A stage input is dealiased before use, so the right-hand side at aliased modes never
+// reaches a kept mode: it only sets the aliased modes of sol at the end of the step, and the
+// next step discards them.  Steps that are not the last of the current call (the step count
+// reaching prune_until) therefore store and update kept modes only; the last step is full.
+template <class RB>
+__device__ __forceinline__ bool step_pruned(const RB& rb) {
+  return rb.clk->step + 1 < rb.clk->prune_until;
 }
 
 // Per-run descriptor read by every kernel (one per run of a batched launch).

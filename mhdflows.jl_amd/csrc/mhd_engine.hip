@@ -140,6 +140,11 @@ class Engine : public EngineBase {
     nr_ = (int)rp.size();
     dev = device;
     MHD_HIP(hipSetDevice(device));
+    {
+      int lds = 0;
+      if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        lds_per_block_limit() = lds;
+    }
     if (s) {
       st = s;
     } else {
@@ -235,6 +240,8 @@ class Engine : public EngineBase {
   static int col_class(int axis, int dir) { return dir > 0 ? (axis == 2 ? 0 : 1) : (axis == 1 ? 2 : 3); }
 
   int gate_ = 0;  // set while a time step is being launched
+  bool prune_enabled = true;  // steps before the last of a call skip the aliased modes (step_pruned)
+  bool host_pruned_ = false;  // byte models of profiled launches: the step being profiled is pruned
 
   // Launch scope: helper launches cover runs [run0_, run0_ + cur_n()) of the group
   // (all runs unless a RunScope narrows them to one, e.g. to set up a single slot).
@@ -254,7 +261,7 @@ class Engine : public EngineBase {
   };
 
   int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
-              bool prune, bool hot = false, bool plane_out = false) {
+              bool prune, bool hot = false, bool plane_out = false, bool prune_out = false) {
     const int n = axis == 1 ? g.ny : g.nz;
     const int S = col_strip<T>(n, hot ? col_var[col_class(axis, dir)] : 0);
     ColArgs a{};
@@ -286,9 +293,17 @@ class Engine : public EngineBase {
     a.nitems = (int64_t)cur_n() * nfields * a.nother * a.nstrips;
     a.gate = gate_;
     a.plane_out = plane_out ? 1 : 0;
+    a.prune_out = prune_out ? 1 : 0;
     const double cxb = sizeof(cx<T>);
-    const double cols = (double)cur_n() * nfields * a.nother * a.ncols;
-    const double bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
+    double bytes;
+    if (prune_out && host_pruned_) {  // pruned step: kept columns (and pencils) only, kept outputs
+      const double ocols = axis == 2 ? (double)nkept_y(g) : (double)a.nother;
+      const double cols = (double)cur_n() * nfields * ocols * g.kx_keep;
+      bytes = cols * n * cxb + cols * (axis == 1 ? nkept_y(g) : nkept_z(g)) * cxb;
+    } else {
+      const double cols = (double)cur_n() * nfields * a.nother * a.ncols;
+      bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
+    }
     const cx<T>* tw = axis == 1 ? twy : twz;
     return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, S, cur_rb(), tw, st); });
   }
@@ -371,12 +386,15 @@ class Engine : public EngineBase {
     std::vector<int> best(5, 0);
     for (int cls = 0; cls < 5; ++cls) {
       float bms = 1e30f;
-      const int nvar = cls < 4 ? 3 : 4;
+      const int nvar = 4;
       std::string line = std::string(names[cls]) + ":";
       for (int v = 0; v < nvar; ++v) {
         if (cls < 4) {
           const int n = (cls == 0 || cls == 3) ? g.nz : g.ny;
-          if (v > 0 && col_strip<T>(n, v) == col_strip<T>(n, 0)) continue;  // not a distinct shape
+          bool dup = false;  // a strip width equal to a lower variant's (or invalid) is not timed again
+          for (int u = 0; u < v; ++u)
+            if (col_strip<T>(n, u) == col_strip<T>(n, v)) dup = true;
+          if (dup) continue;
         } else {
           // row variants: bit 0 plan, bit 1 padded LDS; skip a plan equal to a lower variant's
           bool dup = false;
@@ -416,7 +434,7 @@ class Engine : public EngineBase {
     if (v5) {
       for (int c = 0; c < 4; ++c)
         if (v5[c] >= 0) {
-          if (v5[c] > 2) return fail(E_INVALID, "column-pass variant must be 0..2");
+          if (v5[c] > 3) return fail(E_INVALID, "column-pass variant must be 0..3");
           col_var[c] = v5[c];
         }
       if (v5[4] >= 0) {
@@ -464,10 +482,12 @@ class Engine : public EngineBase {
     ra.write_vars = last ? 1 : 0;
     ra.want_max = last ? 1 : 0;
     const double rows = (double)nr_ * g.ny * g.nz;
-    double bytes = rows * (nf * ra.in_ncols + np * g.nkr) * sizeof(cx<T>) + (last ? rows * nf * g.nx * sizeof(T) : 0);
+    ra.prune_out = gate_;
+    const int kout = host_pruned_ ? g.kx_keep : g.nkr;
+    double bytes = rows * (nf * ra.in_ncols + np * kout) * sizeof(cx<T>) + (last ? rows * nf * g.nx * sizeof(T) : 0);
     MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes, row_var));
-    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true));
-    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));
     const int64_t used_blocks = ra.blocks_per_run;
     sp.g = g;
     sp.nruns = nr_;
@@ -485,7 +505,7 @@ class Engine : public EngineBase {
   double spectral_bytes(const SpecArgs& sp) const {
     const double c = sizeof(cx<T>) * (double)nr_;
     const double kept = (double)g.kx_keep * nkept_y(g) * nkept_z(g);
-    const double all = (double)g.nc;
+    const double all = (sp.gate && host_pruned_) ? kept : (double)g.nc;  // pruned step: kept modes only
     double rd = np * all, wr = 0;
     if (sp.mode == SP_RK4) {
       rd += nf * kept;                                  // stage input
@@ -567,12 +587,21 @@ class Engine : public EngineBase {
 
   int step(int nsteps) override {
     if (nsteps < 0) return fail(E_INVALID, "nsteps must be >= 0");
+    if (nsteps > 0) {  // the last of these steps (per run: or its target) is computed in full
+      const hipError_t e = launch_small<T>(SK_PRUNE, rbd, nr_, phys, nsteps, prune_enabled ? 1 : 0, st);
+      if (e != hipSuccess) return fail(E_HIP, std::string("launch of prune_mark failed: ") + hipGetErrorString(e));
+    }
     if (use_graph && !profiling) {
       MHD_TRY(ensure_graph());
       for (int i = 0; i < nsteps; ++i) MHD_HIP(hipGraphLaunch(gexec, st));
       return 0;
     }
     for (int i = 0; i < nsteps; ++i) MHD_TRY(step_launches());
+    return 0;
+  }
+
+  int set_pruning(int enable) override {
+    prune_enabled = enable != 0;
     return 0;
   }
 
@@ -1079,10 +1108,18 @@ class Engine : public EngineBase {
 
   int profile_steps(int nsteps, std::vector<KTime>* out) override {
     MHD_HIP(hipStreamSynchronize(st));
+    if (nsteps > 0) {
+      const hipError_t e = launch_small<T>(SK_PRUNE, rbd, nr_, phys, nsteps, prune_enabled ? 1 : 0, st);
+      if (e != hipSuccess) return fail(E_HIP, std::string("launch of prune_mark failed: ") + hipGetErrorString(e));
+    }
     profiling = true;
     evs.clear();
     int rc = 0;
-    for (int i = 0; i < nsteps && rc == 0; ++i) rc = step_launches();
+    for (int i = 0; i < nsteps && rc == 0; ++i) {
+      host_pruned_ = prune_enabled && i + 1 < nsteps;  // byte models follow the device's choice
+      rc = step_launches();
+    }
+    host_pruned_ = false;
     profiling = false;
     if (rc) return rc;
     MHD_HIP(hipStreamSynchronize(st));

@@ -59,6 +59,7 @@ class EngineBase {
   virtual int calcN(void* N, void* sol, int advection_only) = 0;
   virtual int component_update(void* N, int field) = 0;
   virtual int step(int nsteps) = 0;
+  virtual int set_pruning(int enable) = 0;
   virtual int cfl_now(double coef, double* dt) = 0;
   virtual int integrate(double t0, int64_t N0, double usr_dt, double coef, int64_t* taken) = 0;
   virtual int set_graph(int enable) = 0;

@@ -43,6 +43,7 @@ struct ColArgs {
   int64_t nitems;
   int gate;            // 1 inside a time step: skip runs whose clock reached its target
   int plane_out;       // inverse z-pass of a masked stage input: copy the kx = 0 column into rb.plane
+  int prune_out;       // forward pass of a time step: in a pruned step only kept outputs are produced
 };
 
 template <class T, int N, int S, int DIR>
@@ -74,8 +75,10 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const int o = oi < a.other_lo_end ? oi : oi + a.other_gap;
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
+  const bool pruned = a.prune_out && step_pruned(rb);
+  if (pruned && (strip * S >= G.kx_keep || (a.axis == 2 && !kept_y(G, o)))) return;  // no kept output
   const int col = strip * S + c;
-  const bool col_ok = col < a.ncols;
+  const bool col_ok = col < a.ncols && !(pruned && col >= G.kx_keep);
   const int64_t base = (int64_t)o * ostride + col;
   const gptr<const cx<T>> __restrict__ src = gmem<const cx<T>>(rb.buf[a.src]) + (int64_t)(a.src_f0 + f) * G.nc + base;
   const gptr<cx<T>> __restrict__ dst = gmem(rb.buf[a.dst]) + (int64_t)(a.dst_f0 + f) * G.nc + base;
@@ -104,6 +107,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
         const int p = g + i * TH + r * (N / RL);
+        if (pruned && !(a.axis == 1 ? kept_y(G, p) : kept_z(G, p))) continue;
         stst<false, T>(dst + p * estride, x[i * RL + r]);
       }
     }
@@ -139,6 +143,7 @@ struct RowArgs {
   int64_t rows;       // ny*nz
   int64_t blocks_per_run;
   int gate;           // 1 inside a time step: skip runs whose clock reached its target
+  int prune_out;      // products of a time step: a pruned step stores the kept kx only
 };
 
 // inputs, products and self-transforms (r2c of the inputs) of each row kind
@@ -195,7 +200,7 @@ template <int M, bool PAD> constexpr int row_slots() { return PAD ? M + 1 + (M +
 template <class T, int M, int E, int TX, class Rfwd, bool WS, bool PAD, class Lds>
 __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>* xrow,
                                               const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twh,
-                                              gptr<cx<T>> __restrict__ dst, bool row_ok) {
+                                              gptr<cx<T>> __restrict__ dst, bool row_ok, int kend) {
   constexpr int RfL = RLast<Rfwd>::value;
   fft_regs<T, M, E, -1, Rfwd, WS>(z, tr, ldsf, twM);
   xsync<WS>();
@@ -204,15 +209,16 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
 #pragma unroll
     for (int r = 0; r < RfL; ++r) xrow[rx<PAD>(tr + i * TX + r * (M / RfL))] = z[i * RfL + r];
   xsync<WS>();
-  // entries k < M; the Nyquist entry M shares Z_0 with k = 0 and is finished there
-  for (int k = tr; k < M; k += TX) {
+  // entries k < min(M, kend); the Nyquist entry M shares Z_0 with k = 0 and is finished there
+  const int kl = kend < M ? kend : M;
+  for (int k = tr; k < kl; k += TX) {
     const cx<T> Zk = xrow[rx<PAD>(k)];
     const cx<T> Zm = conj(xrow[rx<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> Ev = (Zk + Zm) * T(0.5);
     const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
     if (row_ok) {
       stst<false, T>(dst + k, Ev + cmul(twh[k], Od));
-      if (k == 0) stst<false, T>(dst + M, Ev + cmul(twh[M], Od));
+      if (k == 0 && M < kend) stst<false, T>(dst + M, Ev + cmul(twh[M], Od));
     }
   }
 }
@@ -258,6 +264,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   const bool row_ok = row < a.rows;
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
+  const int kend = (a.prune_out && step_pruned(rb)) ? G.kx_keep : M + 1;  // output entries stored
   // The passes read their twiddles from LDS: a table load from the cache hierarchy would
   // sit in every pass's dependency chain.
   for (int i = t; i < 2 * M + 1; i += 128) stw[i] = i < M ? twM[i] : twh[i - M];
@@ -382,7 +389,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       const gptr<cx<T>> __restrict__ dst =
           p < NPROD ? gmem(rb.buf[a.out_sel]) + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
                     : gmem(rb.buf[a.out2_sel]) + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
-      row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, sM, sh, dst, row_ok);
+      row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, sM, sh, dst, row_ok, kend);
     }
   }
 }
@@ -555,6 +562,9 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
       s[j][f] = zero;
       if (j < nmodes && kept[j] && a.self_sel < 0) s[j][f] = ldst<true, T>(in + (int64_t)f * G.nc + qs[j]);
     }
+  }
+  if constexpr (!plane) {
+    if (a.gate && !kept[0] && step_pruned(rb)) return;  // pruned step: aliased modes are not updated
   }
   if (a.zero_aliased) {
     const gptr<cx<T>> __restrict__ io = gmem(rb.buf[a.in_sel]);
@@ -752,6 +762,17 @@ __global__ void advance_clock_kernel(const RunBuf<T>* __restrict__ runs, int nru
   if (c->step >= c->target) return;
   c->t = (double)(T(c->t) + T(c->dt));
   c->step += 1;
+}
+
+// Before a call that advances the runs by nsteps steps: the step that ends the call (or
+// reaches a run's target) is computed in full, the earlier ones are pruned (step_pruned).
+template <class T>
+__global__ void prune_mark_kernel(const RunBuf<T>* __restrict__ runs, int nruns, int64_t nsteps, int enable) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  RunClock* c = runs[r].clk;
+  const int64_t end = c->step + nsteps;
+  c->prune_until = enable ? (end < c->target ? end : c->target) : 0;
 }
 
 // FP64 sums over the real fields: [0] sum |u|^2, [1] sum |b|^2, [2] sum u.b,

@@ -41,14 +41,21 @@ template <class T> inline int64_t row_blocks_max(int nx, int64_t rows) {
   return b;
 }
 
-// column-pass strip widths: variant 0 = one 128 B segment, 1 = half, 2 = double
+// LDS bytes one workgroup may allocate on the device in use (hipDeviceAttributeMaxSharedMemoryPerBlock,
+// read when an engine is created); wider column strips are used only within it.
+inline int64_t& lds_per_block_limit() {
+  static int64_t v = 64 * 1024;
+  return v;
+}
+
+// column-pass strip widths: variant 0 = one 128 B segment, 1 = half, 2 = double, 3 = four
 template <class T> inline int col_strip(int n, int var) {
   const int sw = strip_width<T>();
-  const int s = var == 1 ? sw / 2 : var == 2 ? sw * 2 : sw;
+  const int s = var == 1 ? sw / 2 : var == 2 ? sw * 2 : var == 3 ? sw * 4 : sw;
   const int e = (n == 8 || n == 16 || n == 32 || n == 64 || n == 512) ? 8 : (n % 3 == 0) ? 12 : 16;
   const int threads = s * (n / e);
   const int64_t lds = (int64_t)n * s * (int64_t)sizeof(cx<T>);
-  if (threads > 1024 || lds > 160 * 1024) return sw;
+  if (threads > 1024 || lds > lds_per_block_limit()) return sw;
   return s;
 }
 
@@ -76,6 +83,6 @@ template <class T>
 hipError_t launch_scatter(const Geom& g, cx<T>* x, int nf, int64_t nmodes, const int32_t* idx, const cx<T>* vals,
                           hipStream_t s);
 
-enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3 };
+enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3, SK_PRUNE = 4 };
 
 }  // namespace mhd

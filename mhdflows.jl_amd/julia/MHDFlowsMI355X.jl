@@ -125,6 +125,11 @@ function MHDFlows.getCFL!(prob::MI355XProblem, t_diff; Coef=0.3)
     return dt[]
 end
 
+# Dealiased steps (default on): every step of a call but the last updates kept modes only;
+# the state after the call is the same as with every step computed in full.
+set_step_pruning!(prob::MI355XProblem, enable::Bool) =
+    check(ccall((:mhd_set_step_pruning, libmhd), Int32, (Ptr{Cvoid}, Int32), prob.h, Int32(enable)))
+
 # Operator-level methods: N and sol are device arrays (ROCArray) in the reference layout;
 # only their device pointers cross the boundary.
 devptr(x) = Ptr{Cvoid}(UInt(pointer(x)))

@@ -22,6 +22,7 @@ from .problem import (  # noqa: F401
     nothingfunction,
     refresh_vars,
     set_forcing,
+    set_step_pruning,
     stepforward,
 )
 

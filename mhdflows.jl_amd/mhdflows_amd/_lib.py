@@ -78,6 +78,8 @@ SIGNATURES = {
     "mhd_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),
     "mhd_set_variants": (_I, [_P, _P, _P]),
+    "mhd_set_step_pruning": (_I, [_P, _I]),
+    "mhd_batch_set_step_pruning": (_I, [_P, _I]),
     "mhd_batch_autotune": (_I, [_P, _I]),
     "mhd_profile_steps": (_I, [_P, _I, C.POINTER(KTime), _I, C.POINTER(_I)]),
     "mhd_step_bytes_model": (C.c_double, [C.POINTER(Config)]),

@@ -117,6 +117,9 @@ class Ensemble:
     def sync(self):
         check(lib().mhd_batch_sync(self._h))
 
+    def set_step_pruning(self, enable=True):
+        check(lib().mhd_batch_set_step_pruning(self._h, 1 if enable else 0))
+
     def autotune(self, reps=5):
         check(lib().mhd_batch_autotune(self._h, int(reps)))
 

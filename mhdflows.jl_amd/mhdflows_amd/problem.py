@@ -384,5 +384,11 @@ def variants(prob, set_to=None):
     return list(out)
 
 
+def set_step_pruning(prob, enable=True):
+    """Dealiased steps (see ``mhd_set_step_pruning``): every step of a call but the last
+    updates kept modes only; the state after the call is the same either way."""
+    check(lib().mhd_set_step_pruning(prob._h, 1 if enable else 0))
+
+
 def refresh_vars(prob):
     check(lib().mhd_refresh_vars(prob._h))
