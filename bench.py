@@ -31,16 +31,17 @@ COUNTER_PROFILE = os.path.join("profiles", "mhd256_fp32_rk4_launches.md")
 
 def counter_step_traffic(variants):
     """HBM bytes of one right-hand side's six launches x 4 (an RK4 step) from the committed
-    counter summary, or (None, None) when it was taken with other launch variants."""
+    counter summary (the source string says whether its launch variants match this run's)."""
     try:
         txt = open(os.path.join(ROOT, COUNTER_PROFILE)).read()
         blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
         return None, None
-    if blob.get("variants") != variants:
-        return None, None
     tot = 4 * sum(rec["traffic_bytes"] for rec in blob["launches"].values())
-    return round(tot), f"{COUNTER_PROFILE}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes, same mix of steps"
+    same = "the same launch variants" if blob.get("variants") == variants else \
+        f"launch variants {blob.get('variants')} (this run chose others)"
+    return round(tot), (f"{COUNTER_PROFILE}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes "
+                        f"(dealiased and full steps in the bench's proportion), taken with {same}")
 
 
 def counter_traffic(launch, variants):
@@ -312,6 +313,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-full", action="store_true", help="skip the every-step-full timing")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps profiled per launch (default: --steps, the timed run's mix of steps)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -370,14 +372,16 @@ def main():
     log(f"bench: profiling {nprof} steps with per-launch HIP events")
     prof = profile(M, p, nprof)
     exec_bytes = executed_bytes_per_step(prof, nprof)
-    M.set_step_pruning(p, False)
-    hip().hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    M.TimeIntegrator(p, math.inf, args.steps - 1, CFL_Coef=0.3)
-    hip().hipDeviceSynchronize()
-    wall_full = time.perf_counter() - t0
-    M.set_step_pruning(p, True)
-    log(f"bench: every step full: {args.steps / wall_full:.2f} steps/s")
+    wall_full = None
+    if not args.no_full:
+        M.set_step_pruning(p, False)
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        M.TimeIntegrator(p, math.inf, args.steps - 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        wall_full = time.perf_counter() - t0
+        M.set_step_pruning(p, True)
+        log(f"bench: every step full: {args.steps / wall_full:.2f} steps/s")
     kern = [k for k in prof if k["bytes"] > 0]
     dom = max(kern, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
@@ -391,7 +395,7 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch,
                 "traffic_source": traffic_src}
     step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
-    full_gbs = step_bytes / (wall_full / args.steps) / 1e9
+    full_gbs = step_bytes / (wall_full / args.steps) / 1e9 if wall_full else None
     dram = counter_step_traffic(chosen_named)
     kernels = {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4),
                            "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
@@ -462,7 +466,7 @@ def main():
         "every_step_full": {"value": round(args.steps / wall_full, 3),
                             "ms_per_step": round(wall_full * 1e3 / args.steps, 4),
                             "model_bytes_per_step": step_bytes, "achieved": round(full_gbs, 1),
-                            "frac": round(full_gbs / HBM_PEAK_GBS, 4)},
+                            "frac": round(full_gbs / HBM_PEAK_GBS, 4)} if wall_full else None,
         "kernels": kernels,
         "copy_kernel_GBps": copy_gbs,
         "variants": chosen_named,

@@ -513,6 +513,8 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   const int m = blockIdx.y;
   int l, kx;
   if constexpr (PLANES == 2) {
+    // dealiased step: a slab of aliased m holds no mode to update
+    if (a.gate && !kept_z(G, m) && step_pruned(runs[run])) return;
     if (i >= (uint32_t)(G.ny * G.nkr)) return;
     l = (int)fastdiv(i, G.fd_nkr_m, G.fd_nkr_l);
     kx = (int)i - l * G.nkr;
