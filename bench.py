@@ -29,18 +29,19 @@ HBM_PEAK_GBS = 8000.0
 COUNTER_PROFILE = os.path.join("profiles", "mhd256_fp32_rk4_launches.md")
 
 
-def counter_step_traffic(variants):
+def counter_step_traffic(variants, profile=None):
     """HBM bytes of one right-hand side's six launches x 4 (an RK4 step) from the committed
     counter summary (the source string says whether its launch variants match this run's)."""
+    profile = profile or COUNTER_PROFILE
     try:
-        txt = open(os.path.join(ROOT, COUNTER_PROFILE)).read()
+        txt = open(os.path.join(ROOT, profile)).read()
         blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
         return None, None
     tot = 4 * sum(rec["traffic_bytes"] for rec in blob["launches"].values())
     same = "the same launch variants" if blob.get("variants") == variants else \
         f"launch variants {blob.get('variants')} (this run chose others)"
-    return round(tot), (f"{COUNTER_PROFILE}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes "
+    return round(tot), (f"{profile}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes "
                         f"(dealiased and full steps in the bench's proportion), taken with {same}")
 
 
@@ -66,9 +67,15 @@ def log(*a):
 
 
 def make_problem(M, ic, n, T, seed=2024):
+    """The bench problem; grids above 256^3 get their (|k| <= 16) initial modes set on the
+    device instead of through host real fields of n^3 points."""
     import numpy as np
     p = M.Problem("MI355X", nx=n, ν=1e-3, η=1e-3, B_field=True, T=T, stepper="RK4")
     ms = ic.solenoidal_modes(seed, k0=4.0, kmax=16.0)
+    if n > 256:
+        idx, vals = ms.half_entries(n)
+        M.SetUpProblemIC_modes(p, idx, vals)
+        return p, None
     fields = ms.real_fields(n, dtype=np.float64)
     M.SetUpProblemIC(p, *fields)
     return p, fields
@@ -137,6 +144,9 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
     ms = wall * 1e3 / steps
     full_model = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
     exec_bytes = executed_bytes_per_step(profile(M, p, steps), steps)
+    vn = M.variants(p)
+    dram = counter_step_traffic({"zpass_inv": vn[0], "ypass_inv": vn[1], "ypass_fwd": vn[2], "zpass_fwd": vn[3],
+                                 "xrow_fused": vn[4]}, os.path.join("profiles", "mhd1024_fp32_rk4_launches.md"))
     gbs = full_model / (ms * 1e-3) / 1e9
     d = M.Diagnostics(p)
     M.set_step_pruning(p, False)
@@ -152,7 +162,8 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
            "step_roofline": {"model_bytes_per_step": full_model, "achieved": round(gbs, 1),
                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                              "executed_alg_bytes_per_step": round(exec_bytes),
-                             "executed_alg_GBps": round(exec_bytes / (ms * 1e-3) / 1e9, 1)},
+                             "executed_alg_GBps": round(exec_bytes / (ms * 1e-3) / 1e9, 1),
+                             "dram_bytes_per_step": dram[0], "dram_source": dram[1]},
            "every_step_full": {"value": round(steps / wall_full, 4), "ms_per_step": round(wall_full * 1e3 / steps, 2),
                                "model_bytes_per_step": full_model, "achieved": round(gbs_full, 1),
                                "frac": round(gbs_full / HBM_PEAK_GBS, 4)},
@@ -437,7 +448,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, literal structure)")
-        cpu = cpu_baseline(fields, n, args.cpu_steps)
+        cpu = cpu_baseline(fields, n, args.cpu_steps) if fields is not None else None
 
     out = {
         "metric": f"MHD RK4 steps/s at {n}^3 (Float32)",
