@@ -55,6 +55,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   constexpr int R0 = RFirst<Rad>::value;
   constexpr int RL = RLast<Rad>::value;
   __shared__ cx<T> lds[N * S];
+  __shared__ cx<T> stw[N];  // the twiddle table, staged once per workgroup
 
   const int t = threadIdx.x;
   const int c = t % S;
@@ -93,6 +94,10 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
       x[i * R0 + r] = ok ? ldst<false, T>(src + p * estride) : cx<T>{T(0), T(0)};
     }
   }
+  // twiddles into LDS while the column loads are in flight: the passes then read them
+  // from LDS instead of waiting on the cache hierarchy in their dependency chains
+  for (int i = t; i < N; i += S * TH) stw[i] = tw[i];
+  __syncthreads();
   if (a.plane_out && col == 0) {  // axis 2: o = l, positions p = m
     const gptr<cx<T>> pl = gmem(rb.plane) + ((int64_t)f * G.nz) * G.ny + o;
 #pragma unroll
@@ -100,7 +105,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
 #pragma unroll
       for (int r = 0; r < R0; ++r) pl[(int64_t)(g + i * TH + r * (N / R0)) * G.ny] = x[i * R0 + r];
   }
-  fft_regs<T, N, E, DIR, Rad>(x, g, ldsf, tw);
+  fft_regs<T, N, E, DIR, Rad>(x, g, ldsf, stw);
   if (col_ok) {
 #pragma unroll
     for (int i = 0; i < E / RL; ++i) {

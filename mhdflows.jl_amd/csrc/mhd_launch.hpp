@@ -54,7 +54,7 @@ template <class T> inline int col_strip(int n, int var) {
   const int s = var == 1 ? sw / 2 : var == 2 ? sw * 2 : var == 3 ? sw * 4 : sw;
   const int e = (n == 8 || n == 16 || n == 32 || n == 64 || n == 512) ? 8 : (n % 3 == 0) ? 12 : 16;
   const int threads = s * (n / e);
-  const int64_t lds = (int64_t)n * s * (int64_t)sizeof(cx<T>);
+  const int64_t lds = (int64_t)n * (s + 1) * (int64_t)sizeof(cx<T>);  // tile + twiddle table
   if (threads > 1024 || lds > lds_per_block_limit()) return sw;
   return s;
 }
