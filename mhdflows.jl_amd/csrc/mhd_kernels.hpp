@@ -192,13 +192,15 @@ __device__ __forceinline__ T product(int p, const T* v) {
 template <class T>
 __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
 
-// Row LDS index: with PAD one slot is skipped after every 8, which spreads the
-// strided exchange writes of the first Stockham passes over the LDS banks.
+// Row LDS index.  With PAD the slots of each 16-entry line (128 B of Float32 pairs, all
+// 32 banks) are permuted by an XOR with 5 * line: the strided exchange writes of the
+// Stockham passes (position strides 2 and 4, and blocks of 4 or 8 at stride 16) then
+// spread over the banks, while unit-stride accesses stay conflict-free.
 template <bool PAD>
 __device__ __forceinline__ int rx(int pos) {
-  return PAD ? pos + (pos >> 3) : pos;
+  return PAD ? pos ^ ((5 * (pos >> 4)) & 15) : pos;
 }
-template <int M, bool PAD> constexpr int row_slots() { return PAD ? M + 1 + (M + 1) / 8 + 1 : M + 1; }
+template <int M, bool PAD> constexpr int row_slots() { return PAD ? ((M >> 4) + 1) * 16 : M + 1; }
 
 // Forward real transform of the values z (one complex per pair of real samples,
 // in the pass-0 layout of Rfwd) and store the nkr half-spectrum entries.
@@ -214,16 +216,24 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
 #pragma unroll
     for (int r = 0; r < RfL; ++r) xrow[rx<PAD>(tr + i * TX + r * (M / RfL))] = z[i * RfL + r];
   xsync<WS>();
-  // entries k < min(M, kend); the Nyquist entry M shares Z_0 with k = 0 and is finished there
+  // entries k < min(M, kend); the Nyquist entry M shares Z_0 with k = 0 and is finished there.
+  // The thread's own outputs are the positions k = tr + j*TX (j < E), held in z: Float32
+  // rows take Z_k from there and only the mirror Z_{M-k} through LDS.
+  constexpr int NI = E / RfL;
+  constexpr bool OWN = sizeof(T) == 4;
   const int kl = kend < M ? kend : M;
-  for (int k = tr; k < kl; k += TX) {
-    const cx<T> Zk = xrow[rx<PAD>(k)];
-    const cx<T> Zm = conj(xrow[rx<PAD>(k == 0 ? 0 : M - k)]);
-    const cx<T> Ev = (Zk + Zm) * T(0.5);
-    const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
-    if (row_ok) {
-      stst<false, T>(dst + k, Ev + cmul(twh[k], Od));
-      if (k == 0 && M < kend) stst<false, T>(dst + M, Ev + cmul(twh[M], Od));
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int k = tr + j * TX;
+    if (k < kl) {
+      const cx<T> Zk = OWN ? z[(j % NI) * RfL + j / NI] : xrow[rx<PAD>(k)];
+      const cx<T> Zm = conj(xrow[rx<PAD>(k == 0 ? 0 : M - k)]);
+      const cx<T> Ev = (Zk + Zm) * T(0.5);
+      const cx<T> Od = mul_si<-1>(Zk - Zm) * T(0.5);  // (Zk - Zm) / (2i)
+      if (row_ok) {
+        stst<false, T>(dst + k, Ev + cmul(twh[k], Od));
+        if (k == 0 && M < kend) stst<false, T>(dst + M, Ev + cmul(twh[M], Od));
+      }
     }
   }
 }
@@ -233,7 +243,7 @@ __global__ void __launch_bounds__(128)
 xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
             const cx<T>* __restrict__ twh) {
   constexpr int M = NX / 2;
-  // VAR bit 0 selects the plan (elements per thread), bit 1 the padded LDS index
+  // VAR bit 0 selects the plan (elements per thread), bit 1 the swizzled LDS index
   using RP = RowPlanV<T, M, (VAR & 1)>;
   constexpr bool PAD = (VAR & 2) != 0;
   constexpr int E = RP::E;
@@ -286,21 +296,47 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 
   if constexpr (C2R_IN) {
     const T scale = T(G.inv_n);
+    // Every field's row is requested before the first transform starts, so the loads'
+    // latencies overlap instead of adding up: entries k = tr + i*TX (i < E) land in
+    // v[f][i]; with at least NIN threads per row, lane tr = f holds the Nyquist entry
+    // k = M of field f, otherwise lane 0 loads it when its field is staged.
+    const gptr<const cx<T>> __restrict__ src0 =
+        gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)a.in_f0 * G.nc + row * G.nkr;
+#pragma unroll
+    for (int f = 0; f < NIN; ++f)
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int k = tr + i * TX;
+        v[f][i] = (row_ok && k < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * G.nc + k) : cx<T>{T(0), T(0)};
+      }
+    constexpr bool NYQ_LANES = TX >= NIN;
+    cx<T> nyq = cx<T>{T(0), T(0)};
+    if constexpr (NYQ_LANES)
+      if (row_ok && tr < NIN && M < a.in_ncols) nyq = ldst<true, T>(src0 + (int64_t)tr * G.nc + M);
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
-      const gptr<const cx<T>> __restrict__ src =
-          gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)(a.in_f0 + f) * G.nc + row * G.nkr;
       xsync<WS>();
-      for (int k = tr; k < M; k += TX)
-        xrow[rx<PAD>(k)] = (row_ok && k < a.in_ncols) ? ldst<true, T>(src + k) : cx<T>{T(0), T(0)};
-      if (tr == 0) xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldst<true, T>(src + M) : cx<T>{T(0), T(0)};
+#pragma unroll
+      for (int i = 0; i < E; ++i) xrow[rx<PAD>(tr + i * TX)] = v[f][i];
+      if constexpr (NYQ_LANES) {
+        if (tr == f) xrow[rx<PAD>(M)] = nyq;
+      } else if (tr == 0) {
+        xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * G.nc + M) : cx<T>{T(0), T(0)};
+      }
       xsync<WS>();
+      // Position p = tr + i*TX + r*(M/R0) = tr + (i + r*E/R0)*TX was loaded by this thread
+      // into raw[i + r*E/R0]: Float32 rows take X_p from there and only the mirror X_{M-p}
+      // through LDS (Float64 rows, short of registers, read both from LDS).
+      constexpr bool OWN = sizeof(T) == 4;
+      cx<T> raw[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) raw[e] = v[f][e];
 #pragma unroll
       for (int i = 0; i < E / R0; ++i) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
           const int p = tr + i * TX + r * (M / R0);
-          cx<T> Xk = xrow[rx<PAD>(p)];
+          cx<T> Xk = OWN ? raw[i + r * (E / R0)] : xrow[rx<PAD>(p)];
           cx<T> Xm = xrow[rx<PAD>(M - p)];
           if (p == 0) { Xk.im = T(0); Xm.im = T(0); }  // c2r ignores Im of the DC and Nyquist bins
           const cx<T> A = Xk + conj(Xm);
@@ -373,7 +409,11 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   } else {
     // forward transforms: products (if any) then the inputs themselves (ROW_R2C / ROW_VARS_*)
     constexpr int NOUT = NPROD + RowKindInfo<KIND, NF>::NSELF;
-#pragma unroll 1
+    // Float32 rows unroll this loop (each product's formula is then resolved at compile
+    // time); Float64 rows keep it rolled, as their six resident fields leave no room for
+    // the longer live ranges of the unrolled form.
+    constexpr int UNR = sizeof(T) == 4 ? NOUT : 1;
+#pragma unroll UNR
     for (int p = 0; p < NOUT; ++p) {
       cx<T> z[E];
       if (p < NPROD) {
