@@ -28,7 +28,27 @@ struct Geom {
   double inv_n;    // 1/(nx*ny*nz) in Float64 (rounded to T by the kernels)
   uint32_t fd_nkr_m, fd_nkr_l;  // division by nkr: q = (umulhi(x, m) + x) >> l for x < 2^31
   uint32_t fd_nin_m, fd_nin_l;  // division by nkr - 2 (interior kx count)
+  // The work buffer's z-planes are padded: planes a multiple of 8 KB apart put the nz
+  // elements of a z column on few DRAM channels, and the z passes, which read and write
+  // such columns, then run well below the copy rate.
+  int64_t wps;     // work buffer plane stride (complex elements) = ny*nkr + WORK_PLANE_PAD
+  int64_t wnc;     // work buffer field stride = nz*wps
 };
+
+constexpr int WORK_PLANE_PAD = 32;  // complex elements added to every z-plane of the work buffer
+
+// Plane and field strides of a complex buffer (only the work buffer is padded) and the
+// offset of row (z, y) = row index z*ny + y in it.
+__host__ __device__ inline int64_t plane_stride(const Geom& g, int sel) {
+  return sel == B_WORK ? g.wps : (int64_t)g.ny * g.nkr;
+}
+__host__ __device__ inline int64_t field_stride(const Geom& g, int sel) { return sel == B_WORK ? g.wnc : g.nc; }
+__host__ __device__ inline int64_t row_offset(const Geom& g, int sel, int64_t row) {
+  if (sel != B_WORK) return row * g.nkr;
+  const int r = (int)row;  // rows of one run: ny*nz < 2^31
+  const int z = r / g.ny;
+  return (int64_t)z * g.wps + (int64_t)(r - z * g.ny) * g.nkr;
+}
 
 // Division by an invariant integer d via multiply and shift (l = ceil(log2 d),
 // m = ceil(2^(32+l)/d) - 2^32); exact for 0 <= x < 2^31.
@@ -66,6 +86,8 @@ inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
   aliased_block(nz, &lo, &hi);
   g.mz_lo_end = lo; g.mz_hi_begin = hi;
   g.nc = (int64_t)nz * ny * g.nkr;
+  g.wps = (int64_t)ny * g.nkr + WORK_PLANE_PAD;
+  g.wnc = (int64_t)nz * g.wps;
   g.nr = (int64_t)nz * ny * nx;
   const double twopi = 6.283185307179586;  // Julia's 2π as Float64
   g.multx = ((twopi / Lx) * nx) / nx;

@@ -77,7 +77,7 @@ int64_t run_bytes(const Geom& g, int phys, int prec) {
   const int np = phys == PHYS_MHD ? 9 : 6;
   const int64_t rows = (int64_t)g.ny * g.nz;
   const int64_t blocks = prec == 0 ? row_blocks_max<float>(g.nx, rows) : row_blocks_max<double>(g.nx, rows);
-  return (int64_t)(3 * nf + np) * g.nc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s +
+  return (int64_t)(3 * nf) * g.nc * 2 * s + (int64_t)np * g.wnc * 2 * s + (int64_t)nf * g.nr * s + 2 * blocks * s +
          (int64_t)nf * g.ny * g.nz * 2 * s + 4096;
 }
 
@@ -178,7 +178,7 @@ class Engine : public EngineBase {
       b.rscr = (T*)p;
       MHD_TRY(alloc(cb * nf, &p));
       b.buf[B_ACC] = (cx<T>*)p;
-      MHD_TRY(alloc(cb * np, &p));
+      MHD_TRY(alloc(sizeof(cx<T>) * (size_t)g.wnc * np, &p));
       b.buf[B_WORK] = (cx<T>*)p;
       MHD_TRY(alloc(sizeof(T) * (size_t)g.nr * nf, &p));
       b.vars = (T*)p;
@@ -669,8 +669,10 @@ class Engine : public EngineBase {
     MHD_TRY(r2c_vars_to_work(run));
     for (int f = 0; f < nf; ++f) {
       if (!fields[f]) continue;
-      MHD_HIP(hipMemcpyAsync(b.buf[B_SOL] + (int64_t)f * g.nc, b.buf[B_WORK] + (int64_t)f * g.nc,
-                             sizeof(cx<T>) * g.nc, hipMemcpyDeviceToDevice, st));
+      // work planes are padded: copy plane by plane into the unpadded state layout
+      const size_t pl = sizeof(cx<T>) * (size_t)g.ny * g.nkr;
+      MHD_HIP(hipMemcpy2DAsync(b.buf[B_SOL] + (int64_t)f * g.nc, pl, b.buf[B_WORK] + (int64_t)f * g.wnc,
+                               sizeof(cx<T>) * (size_t)g.wps, pl, (size_t)g.nz, hipMemcpyDeviceToDevice, st));
     }
     MHD_TRY(vars_maxima());
     MHD_HIP(hipStreamSynchronize(st));
@@ -977,7 +979,7 @@ class Engine : public EngineBase {
   int diag_field(int op, int ncomp) {
     for (int r = 0; r < nr_; ++r)
       MHD_TRY(timed("diag", 0, [&] {
-        return launch_diag_spectral<T>(g, rbh[r].buf[B_WORK], rbh[r].buf[B_WORK] + 3 * g.nc, op, st);
+        return launch_diag_spectral<T>(g, rbh[r].buf[B_WORK], rbh[r].buf[B_WORK] + 3 * g.wnc, op, st);
       }));
     MHD_TRY(colpass("zpass_inv", 2, 1, B_WORK, B_WORK, 3, 3, ncomp, false));
     MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 3, 3, ncomp, false));

@@ -61,8 +61,12 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const int c = t % S;
   const int g = t / S;
   const Geom& G = a.g;
-  const int64_t estride = a.axis == 1 ? (int64_t)G.nkr : (int64_t)G.ny * G.nkr;
-  const int64_t ostride = a.axis == 1 ? (int64_t)G.ny * G.nkr : (int64_t)G.nkr;
+  // element (along the axis) and other-index strides of the source and the destination
+  const int64_t sps = plane_stride(G, a.src), dps = plane_stride(G, a.dst);
+  const int64_t s_estride = a.axis == 1 ? (int64_t)G.nkr : sps;
+  const int64_t s_ostride = a.axis == 1 ? sps : (int64_t)G.nkr;
+  const int64_t d_estride = a.axis == 1 ? (int64_t)G.nkr : dps;
+  const int64_t d_ostride = a.axis == 1 ? dps : (int64_t)G.nkr;
   auto ldsf = [&](int pos) -> cx<T>& { return lds[pos * S + c]; };
 
   const int64_t nb = (int64_t)gridDim.x;
@@ -80,9 +84,10 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   if (pruned && (strip * S >= G.kx_keep || (a.axis == 2 && !kept_y(G, o)))) return;  // no kept output
   const int col = strip * S + c;
   const bool col_ok = col < a.ncols && !(pruned && col >= G.kx_keep);
-  const int64_t base = (int64_t)o * ostride + col;
-  const gptr<const cx<T>> __restrict__ src = gmem<const cx<T>>(rb.buf[a.src]) + (int64_t)(a.src_f0 + f) * G.nc + base;
-  const gptr<cx<T>> __restrict__ dst = gmem(rb.buf[a.dst]) + (int64_t)(a.dst_f0 + f) * G.nc + base;
+  const gptr<const cx<T>> __restrict__ src = gmem<const cx<T>>(rb.buf[a.src]) +
+                                             (int64_t)(a.src_f0 + f) * field_stride(G, a.src) + o * s_ostride + col;
+  const gptr<cx<T>> __restrict__ dst =
+      gmem(rb.buf[a.dst]) + (int64_t)(a.dst_f0 + f) * field_stride(G, a.dst) + o * d_ostride + col;
 
   cx<T> x[E];
 #pragma unroll
@@ -91,7 +96,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
     for (int r = 0; r < R0; ++r) {
       const int p = g + i * TH + r * (N / R0);
       const bool ok = col_ok && !(p >= a.mask_lo && p < a.mask_hi);
-      x[i * R0 + r] = ok ? ldst<false, T>(src + p * estride) : cx<T>{T(0), T(0)};
+      x[i * R0 + r] = ok ? ldst<false, T>(src + p * s_estride) : cx<T>{T(0), T(0)};
     }
   }
   // twiddles into LDS while the column loads are in flight: the passes then read them
@@ -113,7 +118,7 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
       for (int r = 0; r < RL; ++r) {
         const int p = g + i * TH + r * (N / RL);
         if (pruned && !(a.axis == 1 ? kept_y(G, p) : kept_z(G, p))) continue;
-        stst<false, T>(dst + p * estride, x[i * RL + r]);
+        stst<false, T>(dst + p * d_estride, x[i * RL + r]);
       }
     }
   }
@@ -300,19 +305,20 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
     // latencies overlap instead of adding up: entries k = tr + i*TX (i < E) land in
     // v[f][i]; with at least NIN threads per row, lane tr = f holds the Nyquist entry
     // k = M of field f, otherwise lane 0 loads it when its field is staged.
+    const int64_t ifs = field_stride(G, a.in_sel);
     const gptr<const cx<T>> __restrict__ src0 =
-        gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)a.in_f0 * G.nc + row * G.nkr;
+        gmem<const cx<T>>(rb.buf[a.in_sel]) + (int64_t)a.in_f0 * ifs + row_offset(G, a.in_sel, row);
 #pragma unroll
     for (int f = 0; f < NIN; ++f)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int k = tr + i * TX;
-        v[f][i] = (row_ok && k < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * G.nc + k) : cx<T>{T(0), T(0)};
+        v[f][i] = (row_ok && k < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * ifs + k) : cx<T>{T(0), T(0)};
       }
     constexpr bool NYQ_LANES = TX >= NIN;
     cx<T> nyq = cx<T>{T(0), T(0)};
     if constexpr (NYQ_LANES)
-      if (row_ok && tr < NIN && M < a.in_ncols) nyq = ldst<true, T>(src0 + (int64_t)tr * G.nc + M);
+      if (row_ok && tr < NIN && M < a.in_ncols) nyq = ldst<true, T>(src0 + (int64_t)tr * ifs + M);
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
       xsync<WS>();
@@ -321,7 +327,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       if constexpr (NYQ_LANES) {
         if (tr == f) xrow[rx<PAD>(M)] = nyq;
       } else if (tr == 0) {
-        xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * G.nc + M) : cx<T>{T(0), T(0)};
+        xrow[rx<PAD>(M)] = (row_ok && M < a.in_ncols) ? ldst<true, T>(src0 + (int64_t)f * ifs + M) : cx<T>{T(0), T(0)};
       }
       xsync<WS>();
       // Position p = tr + i*TX + r*(M/R0) = tr + (i + r*E/R0)*TX was loaded by this thread
@@ -432,8 +438,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
             for (int e = 0; e < E; ++e) z[e] = v[f][e];
       }
       const gptr<cx<T>> __restrict__ dst =
-          p < NPROD ? gmem(rb.buf[a.out_sel]) + (int64_t)(a.out_f0 + p) * G.nc + row * G.nkr
-                    : gmem(rb.buf[a.out2_sel]) + (int64_t)(a.out2_f0 + p - NPROD) * G.nc + row * G.nkr;
+          p < NPROD ? gmem(rb.buf[a.out_sel]) + (int64_t)(a.out_f0 + p) * field_stride(G, a.out_sel) +
+                          row_offset(G, a.out_sel, row)
+                    : gmem(rb.buf[a.out2_sel]) + (int64_t)(a.out2_f0 + p - NPROD) * field_stride(G, a.out2_sel) +
+                          row_offset(G, a.out2_sel, row);
       row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, sM, sh, dst, row_ok, kend);
     }
   }
@@ -497,14 +505,16 @@ __device__ __forceinline__ T ipow(T x, int n) {
   return r;
 }
 
+// qp: the mode's offset in the products buffer (its planes may be padded)
 template <class T, int PHYS>
 __device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const SpecArgs& a,
-                                       int64_t q, const ModeK<T>& k, const cx<T>* sH, cx<T>* R) {
+                                       int64_t qp, const ModeK<T>& k, const cx<T>* sH, cx<T>* R) {
   constexpr int NP = PHYS == PHYS_MHD ? 9 : 6;
   const gptr<const cx<T>> __restrict__ P = gmem<const cx<T>>(rb.buf[a.prod_sel]);
+  const int64_t pfs = field_stride(G, a.prod_sel);
   cx<T> p[NP];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) p[i] = ldst<true, T>(P + (int64_t)i * G.nc + q);
+  for (int i = 0; i < NP; ++i) p[i] = ldst<true, T>(P + (int64_t)i * pfs + qp);
   const cx<T> zero{T(0), T(0)};
   if constexpr (PHYS == PHYS_MHD) {
     // D_j = sum_i i k_i T_ij ; N_u = D - k (k.D)/k^2  (UᵢUpdate!, MHDSolver.jl:63-83)
@@ -651,7 +661,8 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
       for (int f = 0; f < NF; ++f) sH[f] = s[j][f];
     }
     cx<T> R[NF];
-    rhs_at<T, PHYS>(G, rb, a, q, k, sH, R);
+    const int64_t qp = q + (plane_stride(G, a.prod_sel) - (int64_t)G.ny * G.nkr) * ms[j];
+    rhs_at<T, PHYS>(G, rb, a, qp, k, sH, R);
     if (PHYS == PHYS_MHD && a.add_force && rb.force) {
       const int K = rb.force_K;
       const int sl = signed_index(ls[j], G.ny), sm = signed_index(ms[j], G.nz);
@@ -892,16 +903,18 @@ dot_max_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t nr, int
 // (fields 0..2 of src) written to dst fields 0..2 (curl / vector potential) or
 // field 0 (divergence).  Wavenumbers in FP64.
 enum DiagOp { OP_CURL = 0, OP_VECPOT = 1, OP_DIV = 2 };
+// src and dst are fields of the work buffer (padded planes: field stride wnc, plane stride wps).
 template <class T>
 __global__ void __launch_bounds__(256)
 diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int op) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= G.nc) return;
-  const int kx = (int)(idx % G.nkr);
-  const int64_t rest = idx / G.nkr;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= G.nc) return;
+  const int kx = (int)(i % G.nkr);
+  const int64_t rest = i / G.nkr;
   const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.nkr + kx;
   const double k0 = kx * G.multx, k1 = signed_index(l, G.ny) * G.multy, k2 = signed_index(m, G.nz) * G.multz;
-  const cx<T> a0 = src[idx], a1 = src[G.nc + idx], a2 = src[2 * G.nc + idx];
+  const cx<T> a0 = src[idx], a1 = src[G.wnc + idx], a2 = src[2 * G.wnc + idx];
   auto ik = [](double kk, cx<T> z) { return cx<T>{T(-kk * (double)z.im), T(kk * (double)z.re)}; };
   if (op == OP_DIV) {
     const cx<T> d = ik(k0, a0) + ik(k1, a1) + ik(k2, a2);
@@ -916,7 +929,7 @@ diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ 
   const cx<T> c0 = (ik(k1, a2) - ik(k2, a1)) * T(s);
   const cx<T> c1 = (ik(k2, a0) - ik(k0, a2)) * T(s);
   const cx<T> c2 = (ik(k0, a1) - ik(k1, a0)) * T(s);
-  dst[idx] = c0; dst[G.nc + idx] = c1; dst[2 * G.nc + idx] = c2;
+  dst[idx] = c0; dst[G.wnc + idx] = c1; dst[2 * G.wnc + idx] = c2;
 }
 
 // scatter sparse half-spectrum entries (kx, l, m) of nf fields into x (already zeroed)
