@@ -523,7 +523,9 @@ class Engine : public EngineBase {
     const double rows = (double)nr_ * g.ny * g.nz;
     ra.prune_out = gate_;
     const int kout = host_pruned_ ? g.kx_keep : g.nkr;
-    double bytes = rows * (nf * ra.in_ncols + np * kout) * sizeof(cx<T>) + (last ? rows * nf * g.nx * sizeof(T) : 0);
+    // the real fields are stored at the last stage of a full step only
+    double bytes = rows * (nf * ra.in_ncols + np * kout) * sizeof(cx<T>) +
+                   (last && !host_pruned_ ? rows * nf * g.nx * sizeof(T) : 0);
     MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes, row_var));
     MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));
     MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));

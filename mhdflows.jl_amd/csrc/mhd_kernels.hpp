@@ -354,7 +354,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 #pragma unroll
       for (int e = 0; e < E; ++e) v[f][e] = v[f][e] * scale;
     }
-    if (a.write_vars && row_ok) {
+    // A dealiased step's real fields would be overwritten by a later step of the same call
+    // before anything reads them (the CFL takes the maxima below, not vars), so only the
+    // call's last step, which is full, stores them.
+    if (a.write_vars && row_ok && !(a.prune_out && step_pruned(rb))) {
 #pragma unroll
       for (int f = 0; f < NIN; ++f) {
         const gptr<T> __restrict__ vr = rbase + (int64_t)(a.real_out_f0 + f) * G.nr + row * NX;

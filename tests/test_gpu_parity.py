@@ -294,6 +294,9 @@ def test_step_pruning_gives_the_same_state(T, physics, stepper, n):
         M.stepforward(a, k)
         M.stepforward(b, k)
         assert np.array_equal(a.sol, b.sol), k
+        # prob.vars (stored by each call's last, full step only) match as well
+        for f in range(a.sol.shape[0]):
+            assert np.array_equal(a.vars.field(f), b.vars.field(f)), (k, f)
     if T == np.float64 and stepper == "RK4":
         run_steps(o, 9)
         assert rel_l2(a.sol, o.sol) < 1e-12
