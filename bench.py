@@ -341,7 +341,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (single-GPU build; "
-              "see DESIGN.md for how a run would shard)", file=sys.stderr)
+              "a single large run would shard as slabs in z with one all-to-all per 3D transform, "
+              "ensembles by placing runs on GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
