@@ -757,10 +757,29 @@ reduce_max_kernel(const RunBuf<T>* __restrict__ runs, int64_t nblocks, int advan
   const RunBuf<T>& rb = runs[blockIdx.x];
   if (gated_off(advance, rb)) return;
   const int t = threadIdx.x;
-  T mu = T(0), mb = T(0);
-  for (int64_t i = t; i < nblocks; i += 256) {
-    mu = nanmax(mu, rb.partial[2 * i]);
-    mb = nanmax(mb, rb.partial[2 * i + 1]);
+  // eight independent accumulators keep eight loads in flight per thread (the maximum,
+  // NaN included, does not depend on the order)
+  constexpr int K = 8;
+  T mus[K], mbs[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) mus[k] = mbs[k] = T(0);
+  int64_t i = t;
+  for (; i + (K - 1) * 256 < nblocks; i += K * 256) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      mus[k] = nanmax(mus[k], rb.partial[2 * (i + k * 256)]);
+      mbs[k] = nanmax(mbs[k], rb.partial[2 * (i + k * 256) + 1]);
+    }
+  }
+  for (; i < nblocks; i += 256) {
+    mus[0] = nanmax(mus[0], rb.partial[2 * i]);
+    mbs[0] = nanmax(mbs[0], rb.partial[2 * i + 1]);
+  }
+  T mu = mus[0], mb = mbs[0];
+#pragma unroll
+  for (int k = 1; k < K; ++k) {
+    mu = nanmax(mu, mus[k]);
+    mb = nanmax(mb, mbs[k]);
   }
   red[0][t] = mu;
   red[1][t] = mb;
