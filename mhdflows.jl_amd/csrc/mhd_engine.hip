@@ -344,12 +344,24 @@ class Engine : public EngineBase {
 
   // time `reps` launches of one hot kernel with the given variant (work/stin only)
   int time_variant(int cls, int var, int reps, float* ms) {
-    const int save_col = cls < 4 ? col_var[cls] : 0, save_row = row_var;
+    // The variant under test is set for this call only; the previous choice is restored and
+    // the events are released on every return path (error returns included).
+    struct Guard {
+      Engine* e;
+      int cls, col, row;
+      hipEvent_t a = nullptr, b = nullptr;
+      ~Guard() {
+        if (cls < 4) e->col_var[cls] = col;
+        else e->row_var = row;
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+      }
+    } gd{this, cls, cls < 4 ? col_var[cls] : 0, row_var};
     if (cls < 4) col_var[cls] = var;
     else row_var = var;
-    hipEvent_t a, b;
-    MHD_HIP(hipEventCreate(&a));
-    MHD_HIP(hipEventCreate(&b));
+    MHD_HIP(hipEventCreate(&gd.a));
+    MHD_HIP(hipEventCreate(&gd.b));
+    const hipEvent_t a = gd.a, b = gd.b;
     int rc = 0;
     if (cls == 4) {
       // The row kernel is timed where a step runs it: right after the inverse passes
@@ -374,9 +386,6 @@ class Engine : public EngineBase {
         if (k > 0) sum += e;  // the first launch warms up
       }
       *ms = sum;
-      hipEventDestroy(a);
-      hipEventDestroy(b);
-      row_var = save_row;
       return rc;
     }
     for (int k = 0; k <= reps && rc == 0; ++k) {
@@ -399,10 +408,6 @@ class Engine : public EngineBase {
     MHD_HIP(hipEventRecord(b, st));
     MHD_HIP(hipEventSynchronize(b));
     MHD_HIP(hipEventElapsedTime(ms, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    if (cls < 4) col_var[cls] = save_col;
-    else row_var = save_row;
     return rc;
   }
 
