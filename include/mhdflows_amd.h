@@ -176,9 +176,10 @@ int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp);
 int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len);
 /* Set (v5 != NULL; negative entries keep the current value) and/or read (out5 != NULL)
  * the launch variants {zpass_inv, ypass_inv, ypass_fwd, zpass_fwd: strip width
- * 0 = 128 B, 1 = 64 B, 2 = 256 B, 3 = 512 B of kx per row; xrow_fused: bit 0 plan, bit 1 padded
- * LDS (0..3)}.  Results do not depend on
- * the variant beyond floating-point rounding of the transform order. */
+ * 0 = 128 B, 1 = 64 B, 2 = 256 B, 3 = 512 B of kx per row; xrow_fused: bit 0 plan (Float32
+ * powers of two: 0 = eight, 1 = four elements per thread), bit 1 XOR-swizzled LDS index
+ * (0..3)}.  Results do not depend on the variant beyond floating-point rounding of the
+ * transform order. */
 int32_t mhd_set_variants(mhd_handle* h, const int32_t* v5, int32_t* out5);
 /* Time nsteps steps with HIP events on the handle's stream (whole-step wall time). */
 int32_t mhd_time_steps(mhd_handle* h, int32_t nsteps, double* ms_out);
