@@ -71,7 +71,9 @@ typedef struct mhd_diag {
   double E_kin;      /* Σ|u|² dV (no 1/2, as the reference) */
   double E_mag;      /* Σ|b|² dV */
   double H_kin;      /* Σ u·(∇×u) dV */
-  double H_mag;      /* Σ A·b dV, Coulomb gauge */
+  double H_mag;      /* Σ A·b dV, Coulomb gauge: the volume integral; the reference's h_m
+                        (MHDAnalysis.jl:113-117) returns the density A·b without dV, so
+                        sum(h_m) = H_mag / dV, while sum(h_k) = H_kin (h_k includes dV) */
   double H_cross;    /* Σ u·b dV */
   double max_div_b;  /* max |∇·b| (Div, src/utils/VectorCalculus.jl:72-94) */
   double max_div_u;  /* max |∇·u| */
@@ -111,6 +113,11 @@ int32_t mhd_set_sol(mhd_handle* h, const void* host_in);
  * array indices per entry, values holds nf complex T per entry (field-fastest).
  * vars are refreshed from the dealiased state (a spectral SetUpProblemIC!). */
 int32_t mhd_set_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values);
+/* Read nmodes entries of prob.sol: idx holds (kx, l, m) array indices per entry, values_out
+ * receives nf complex T per entry (field-fastest).  A partial read of the state (no
+ * counterpart in the reference, which copies whole arrays), e.g. the modes a coarser grid
+ * keeps, without moving the whole half spectrum over PCIe. */
+int32_t mhd_get_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, void* values_out);
 /* Copy prob.vars real field f (0..nf-1) to host memory. */
 int32_t mhd_get_vars(mhd_handle* h, int32_t field, void* host_out);
 /* Device pointers of the handle's sol (nf fields) and vars (nf real fields). */
@@ -196,13 +203,14 @@ double mhd_step_bytes_model(const mhd_config* cfg);
 /* ---- batched ensembles: many independent runs stepped together ----------------- */
 typedef struct mhd_batch mhd_batch;
 /* Runs are grouped by (grid size, precision); each group steps with one launch per
- * kernel class covering all its runs, and the groups' launches are captured into
- * one HIP graph per ensemble step. */
+ * kernel class covering all its runs (one captured HIP graph per group step), and the
+ * groups step concurrently on their own streams. */
 int32_t mhd_batch_create(const mhd_config* cfgs, int32_t nruns, mhd_batch** out);
 int32_t mhd_batch_destroy(mhd_batch* b);
 int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields);
 int32_t mhd_batch_set_sol(mhd_batch* b, int32_t run, const void* host_in);
 int32_t mhd_batch_set_sol_modes(mhd_batch* b, int32_t run, int64_t nmodes, const int32_t* idx, const void* values);
+int32_t mhd_batch_get_sol_modes(mhd_batch* b, int32_t run, int64_t nmodes, const int32_t* idx, void* values_out);
 int32_t mhd_batch_get_sol(mhd_batch* b, int32_t run, void* host_out);
 int32_t mhd_batch_get_clock(mhd_batch* b, int32_t run, double* t, double* dt, int64_t* step);
 /* Set per-run CFL control (coef > 0: dt from getCFL! before every step). */

@@ -116,6 +116,7 @@ class Engine : public EngineBase {
     for (void* p : force_tabs)
       if (p) hipFree(p);
     if (dpart) hipFree(dpart);
+    if (stage_) hipFree(stage_);
     if (rbd) hipFree(rbd);
     if (own_stream && st) hipStreamDestroy(st);
   }
@@ -721,31 +722,73 @@ class Engine : public EngineBase {
     MHD_HIP(hipStreamSynchronize(st));
     return 0;
   }
-  // zero sol, scatter sparse modes, then refresh vars (and the CFL maxima) from it
-  int set_sol_modes(int run, int64_t nmodes, const int32_t* idx, const void* vals) override {
-    MHD_TRY(check_run(run));
+  // ---- sparse half-spectrum entries ----------------------------------------------
+  // Device staging of (kx, l, m) index triples and values: idx first (padded to 256 B),
+  // then nf complex T per entry.  The engine keeps one buffer that grows on demand.
+  void* stage_ = nullptr;
+  size_t stage_cap_ = 0;
+  static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+  size_t mode_stage_bytes(int64_t nmodes) const override {
+    return align256(sizeof(int32_t) * 3 * (size_t)nmodes) + sizeof(cx<T>) * (size_t)nf * (size_t)nmodes;
+  }
+  int grow_stage(size_t bytes) {
+    if (bytes <= stage_cap_) return 0;
+    MHD_HIP(hipStreamSynchronize(st));
+    if (stage_) hipFree(stage_);
+    stage_ = nullptr;
+    stage_cap_ = 0;
+    MHD_HIP(hipMalloc(&stage_, bytes));
+    stage_cap_ = bytes;
+    return 0;
+  }
+  int check_modes(int64_t nmodes, const int32_t* idx) const {
     if (nmodes < 0) return fail(E_INVALID, "nmodes must be >= 0");
     for (int64_t i = 0; i < nmodes; ++i) {
       if (idx[3 * i] < 0 || idx[3 * i] >= g.nkr || idx[3 * i + 1] < 0 || idx[3 * i + 1] >= g.ny ||
           idx[3 * i + 2] < 0 || idx[3 * i + 2] >= g.nz)
         return fail(E_INVALID, "mode index outside the half spectrum");
     }
+    return 0;
+  }
+
+  // zero sol, scatter sparse modes, then refresh vars (and the CFL maxima) from it
+  int set_sol_modes(int run, int64_t nmodes, const int32_t* idx, const void* vals) override {
+    MHD_TRY(check_run(run));
+    MHD_TRY(check_modes(nmodes, idx));
+    if (nmodes > 0) MHD_TRY(grow_stage(mode_stage_bytes(nmodes)));
+    return set_sol_modes_staged(run, nmodes, idx, vals, stage_, stage_cap_);
+  }
+
+  int set_sol_modes_staged(int run, int64_t nmodes, const int32_t* idx, const void* vals, void* stage,
+                           size_t cap) override {
+    MHD_TRY(check_run(run));
+    MHD_TRY(check_modes(nmodes, idx));
+    if (nmodes > 0 && (!stage || mode_stage_bytes(nmodes) > cap))
+      return fail(E_INVALID, "staging buffer too small for " + std::to_string(nmodes) + " modes");
     RunBuf<T>& b = rbh[run];
-    MHD_HIP(hipMemsetAsync(b.buf[B_SOL], 0, sizeof(cx<T>) * g.nc * nf, st));
+    MHD_TRY(timed("zero", 0, [&] { return launch_zero<T>(b.buf[B_SOL], g.nc * nf, st); }));
     if (nmodes > 0) {
-      void *di = nullptr, *dv = nullptr;
-      MHD_HIP(hipMalloc(&di, sizeof(int32_t) * 3 * nmodes));
-      MHD_HIP(hipMalloc(&dv, sizeof(cx<T>) * nf * nmodes));
+      int32_t* di = (int32_t*)stage;
+      cx<T>* dv = (cx<T>*)((char*)stage + align256(sizeof(int32_t) * 3 * (size_t)nmodes));
       MHD_HIP(hipMemcpyAsync(di, idx, sizeof(int32_t) * 3 * nmodes, hipMemcpyHostToDevice, st));
       MHD_HIP(hipMemcpyAsync(dv, vals, sizeof(cx<T>) * nf * nmodes, hipMemcpyHostToDevice, st));
-      MHD_TRY(timed("scatter", 0, [&] {
-        return launch_scatter<T>(g, b.buf[B_SOL], nf, nmodes, (const int32_t*)di, (const cx<T>*)dv, st);
-      }));
-      MHD_HIP(hipStreamSynchronize(st));
-      hipFree(di);
-      hipFree(dv);
+      MHD_TRY(timed("scatter", 0, [&] { return launch_scatter<T>(g, b.buf[B_SOL], nf, nmodes, di, dv, st); }));
     }
-    return refresh_vars_run(run);
+    return refresh_vars_run(run);  // ends with a stream synchronisation: the staging is free again
+  }
+
+  int get_sol_modes(int run, int64_t nmodes, const int32_t* idx, void* vals) override {
+    MHD_TRY(check_run(run));
+    MHD_TRY(check_modes(nmodes, idx));
+    if (nmodes == 0) return 0;
+    MHD_TRY(grow_stage(mode_stage_bytes(nmodes)));
+    int32_t* di = (int32_t*)stage_;
+    cx<T>* dv = (cx<T>*)((char*)stage_ + align256(sizeof(int32_t) * 3 * (size_t)nmodes));
+    MHD_HIP(hipMemcpyAsync(di, idx, sizeof(int32_t) * 3 * nmodes, hipMemcpyHostToDevice, st));
+    MHD_TRY(timed("gather", 0, [&] { return launch_gather<T>(g, rbh[run].buf[B_SOL], nf, nmodes, di, dv, st); }));
+    MHD_HIP(hipMemcpyAsync(vals, dv, sizeof(cx<T>) * nf * nmodes, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
   }
 
   int get_vars(int run, int f, void* host) override {
@@ -1137,19 +1180,23 @@ class Engine : public EngineBase {
 
   // ---- measurement -------------------------------------------------------------
   int time_steps(int nsteps, double* ms) override {
-    hipEvent_t a, b;
-    MHD_HIP(hipEventCreate(&a));
-    MHD_HIP(hipEventCreate(&b));
+    struct Pair {  // released on every return path
+      hipEvent_t a = nullptr, b = nullptr;
+      ~Pair() {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+      }
+    } ev;
+    MHD_HIP(hipEventCreate(&ev.a));
+    MHD_HIP(hipEventCreate(&ev.b));
     if (use_graph) MHD_TRY(ensure_graph());
     MHD_HIP(hipStreamSynchronize(st));
-    MHD_HIP(hipEventRecord(a, st));
+    MHD_HIP(hipEventRecord(ev.a, st));
     MHD_TRY(step(nsteps));
-    MHD_HIP(hipEventRecord(b, st));
-    MHD_HIP(hipEventSynchronize(b));
+    MHD_HIP(hipEventRecord(ev.b, st));
+    MHD_HIP(hipEventSynchronize(ev.b));
     float f = 0;
-    MHD_HIP(hipEventElapsedTime(&f, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    MHD_HIP(hipEventElapsedTime(&f, ev.a, ev.b));
     *ms = f;
     return 0;
   }
@@ -1169,7 +1216,14 @@ class Engine : public EngineBase {
     }
     host_pruned_ = false;
     profiling = false;
-    if (rc) return rc;
+    if (rc) {
+      for (auto& e : evs) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+      }
+      evs.clear();
+      return rc;
+    }
     MHD_HIP(hipStreamSynchronize(st));
     std::map<std::string, KTime> agg;
     std::vector<std::string> order;

@@ -51,6 +51,13 @@ class EngineBase {
   virtual int get_sol(int run, void* host) = 0;
   virtual int set_sol(int run, const void* host) = 0;
   virtual int set_sol_modes(int run, int64_t nmodes, const int32_t* idx, const void* vals) = 0;
+  // set_sol_modes with the device staging of idx and vals in a caller's buffer of cap bytes
+  // (at least mode_stage_bytes(nmodes)); allocates nothing
+  virtual int set_sol_modes_staged(int run, int64_t nmodes, const int32_t* idx, const void* vals, void* stage,
+                                   size_t cap) = 0;
+  virtual size_t mode_stage_bytes(int64_t nmodes) const = 0;
+  // read nmodes half-spectrum entries of sol (field-fastest, nf complex T per entry)
+  virtual int get_sol_modes(int run, int64_t nmodes, const int32_t* idx, void* vals) = 0;
   virtual int get_vars(int run, int f, void* host) = 0;
   virtual int device_ptrs(int run, void** sol, void** vars) = 0;
   virtual int get_clock(int run, double* t, double* dt, int64_t* step) = 0;

@@ -967,6 +967,28 @@ scatter_modes_kernel(Geom G, cx<T>* __restrict__ x, int nf, int64_t nmodes, cons
   for (int f = 0; f < nf; ++f) x[(int64_t)f * G.nc + q] = vals[i * nf + f];
 }
 
+// gather sparse half-spectrum entries (kx, l, m) of nf fields of x into vals (field-fastest);
+// entries outside the half spectrum read as zero
+template <class T>
+__global__ void __launch_bounds__(256)
+gather_modes_kernel(Geom G, const cx<T>* __restrict__ x, int nf, int64_t nmodes, const int32_t* __restrict__ idx,
+                    cx<T>* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nmodes) return;
+  const int kx = idx[3 * i], l = idx[3 * i + 1], m = idx[3 * i + 2];
+  const bool ok = kx >= 0 && kx < G.nkr && l >= 0 && l < G.ny && m >= 0 && m < G.nz;
+  const int64_t q = ok ? ((int64_t)m * G.ny + l) * G.nkr + kx : 0;
+  for (int f = 0; f < nf; ++f) vals[i * nf + f] = ok ? x[(int64_t)f * G.nc + q] : cx<T>{T(0), T(0)};
+}
+
+// zero n complex values (a fill kernel: usable where a memset node is not wanted)
+template <class T>
+__global__ void __launch_bounds__(256)
+zero_kernel(cx<T>* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    x[i] = cx<T>{T(0), T(0)};
+}
+
 // zero the aliased band of nf fields in place (FourierFlows dealias!)
 template <class T>
 __global__ void __launch_bounds__(256)

@@ -82,6 +82,11 @@ hipError_t launch_dealias(const Geom& g, cx<T>* x, int nf, hipStream_t s);
 template <class T>
 hipError_t launch_scatter(const Geom& g, cx<T>* x, int nf, int64_t nmodes, const int32_t* idx, const cx<T>* vals,
                           hipStream_t s);
+template <class T>
+hipError_t launch_gather(const Geom& g, const cx<T>* x, int nf, int64_t nmodes, const int32_t* idx, cx<T>* vals,
+                         hipStream_t s);
+template <class T>
+hipError_t launch_zero(cx<T>* x, int64_t n, hipStream_t s);
 
 enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3, SK_PRUNE = 4 };
 

@@ -19,6 +19,7 @@ from .problem import (  # noqa: F401
     UiUpdate,
     calcN,
     getCFL,
+    get_sol_modes,
     nothingfunction,
     refresh_vars,
     set_forcing,
@@ -29,5 +30,5 @@ from .problem import (  # noqa: F401
 __all__ = [
     "Problem", "SetUpProblemIC", "SetUpProblemIC_modes", "stepforward", "TimeIntegrator", "getCFL", "calcN",
     "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
-    "Diagnostics", "set_forcing", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
+    "Diagnostics", "set_forcing", "get_sol_modes", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
 ]

@@ -83,6 +83,15 @@ class Ensemble:
         ptrs = (C.c_void_p * nf)(*[a.ctypes.data for a in arrs])
         check(lib().mhd_batch_set_ic_real(self._h, run, ptrs))
 
+    def get_sol_modes(self, run, idx):
+        """Entries ``(kx, l, m)`` (rows of ``idx``) of run ``run``'s state, shape ``(M, nf)``."""
+        c, T, Cx, nf = self._types(run)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.empty((idx.shape[0], nf), Cx)
+        check(lib().mhd_batch_get_sol_modes(self._h, run, idx.shape[0], idx.ctypes.data_as(C.c_void_p),
+                                            out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def get_sol(self, run):
         c, T, Cx, nf = self._types(run)
         out = np.empty((nf, c.nz, c.ny, c.nx // 2 + 1), Cx)

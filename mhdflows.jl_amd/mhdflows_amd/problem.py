@@ -204,6 +204,11 @@ def Problem(dev="MI355X", **kw):
             raise MHDError(-2, f"{flag} is not part of the MI355X hot path")
     stepper = _pick(kw, ["stepper"], "RK4")
     calcF = _pick(kw, ["calcF"], nothingfunction)
+    if calcF is not nothingfunction and not getattr(calcF, "_device_forcing", False):
+        # addforcing! calls calcF! inside every right-hand side (src/pgen.jl:231-234); a host
+        # callable cannot run inside the device step, so it is refused rather than dropped
+        raise MHDError(-2, "a user calcF! cannot run inside the device step; use the library's "
+                           "forcing modules instead")
     T = _pick(kw, ["T"], np.float32)
     dt = float(_pick(kw, ["dt"], 0.0))
     _pick(kw, ["aliased_fraction"], 1 / 3)
@@ -252,6 +257,17 @@ def SetUpProblemIC_modes(prob, idx, vals):
         raise MHDError(-1, "idx must be (M,3) and vals (M,nf)")
     check(lib().mhd_set_sol_modes(prob._h, idx.shape[0], idx.ctypes.data_as(C.c_void_p),
                                   vals.ctypes.data_as(C.c_void_p)))
+
+
+def get_sol_modes(prob, idx):
+    """Entries ``(kx, l, m)`` (rows of ``idx``) of ``prob.sol``, shape ``(M, nf)``."""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    if idx.ndim != 2 or idx.shape[1] != 3:
+        raise MHDError(-1, "idx must be (M,3)")
+    out = np.empty((idx.shape[0], prob.nf), dtype=prob.C)
+    check(lib().mhd_get_sol_modes(prob._h, idx.shape[0], idx.ctypes.data_as(C.c_void_p),
+                                  out.ctypes.data_as(C.c_void_p)))
+    return out
 
 
 def stepforward(prob, nsteps=1):

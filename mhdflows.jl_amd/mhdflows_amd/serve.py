@@ -20,6 +20,7 @@ import numpy as np
 
 from . import ensemble as ens
 from . import ic
+from ._lib import MHDError
 
 
 @dataclasses.dataclass
@@ -46,6 +47,7 @@ class Result:
     admitted: float
     finished: float
     sol: np.ndarray = None
+    error: str = None       # set when the request was refused (never admitted)
 
     @property
     def latency(self):
@@ -87,13 +89,30 @@ class Server:
         self.used = 0
         self.steps_done = 0
 
+    def _pool_bytes(self, n, T):
+        """Device bytes of the slot pool for (n, T); raises MHDError for an unsupported grid."""
+        return self.slots.get(n, 2) * ens.query_bytes(ens.make_config(n, T=T, stepper=self.stepper))
+
+    def rejection(self, req):
+        """Why ``req`` can never be served (unsupported configuration, or a slot pool larger
+        than the whole HBM budget), or None.  Such requests are refused on arrival instead
+        of waiting for memory that can never become free."""
+        try:
+            need = self._pool_bytes(req.n, req.T)
+        except MHDError as e:
+            return str(e)
+        if need > self.budget:
+            return (f"the {req.n}^3 {np.dtype(req.T).name} slot pool needs {need} bytes, "
+                    f"more than the HBM budget of {self.budget}")
+        return None
+
     def _pool(self, n, T):
         key = (n, np.dtype(T).name)
         p = self.pools.get(key)
         if p is not None:
             return p
         slots = self.slots.get(n, 2)
-        need = slots * ens.query_bytes(ens.make_config(n, T=T, stepper=self.stepper))
+        need = self._pool_bytes(n, T)
         if self.used + need > self.budget:
             # release idle pools until the new one fits
             for k in list(self.pools):
@@ -132,7 +151,14 @@ class Server:
         while pending or waiting or any(p.active for p in self.pools.values()):
             now = time.perf_counter() - t0
             while pending and (pending[0].arrival <= now or not realtime):
-                waiting.append(pending.popleft())
+                r = pending.popleft()
+                why = self.rejection(r)
+                if why is None:
+                    waiting.append(r)
+                else:
+                    results[r.rid] = Result(r.rid, r.n, r.T, r.steps, r.arrival, now, now, None, why)
+                    if log:
+                        log(f"refused {r.rid}: {why}")
             # admit in arrival order; a request that cannot be placed does not block
             # requests of other sizes behind it
             blocked = set()
@@ -193,8 +219,10 @@ def poisson_stream(nreq, rate, seed=0, steps=(100, 2000), sizes=(32, 64, 128, 25
 
 
 def summarize(results, makespan):
-    lat = np.array([r.latency for r in results.values()])
-    steps = sum(r.steps for r in results.values())
-    return {"requests": len(results), "p50_latency_s": float(np.percentile(lat, 50)),
+    done = [r for r in results.values() if r.error is None]
+    lat = np.array([r.latency for r in done]) if done else np.zeros(1)
+    steps = sum(r.steps for r in done)
+    return {"requests": len(done), "refused": len(results) - len(done),
+            "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "aggregate_steps_per_s": steps / makespan,
             "makespan_s": makespan}

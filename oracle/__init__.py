@@ -37,6 +37,11 @@ How each part is pinned (checked by ``tests/test_oracle_*.py``)
   polynomial), divergence of the right-hand side at round-off, ideal-MHD energy
   and cross-helicity conservation to O(dt^4).
 * Literal vs fused modes agree to round-off (two independent op structures).
+* HD sign: the HD right-hand side equals the MHD velocity right-hand side with b = 0
+  (whose u u sign cross-helicity conservation fixes), and the 3D Taylor-Green field's
+  t = 0 right-hand side matches its closed form (pressure (cos 2x + cos 2y)(2 + cos 2z)/16).
+* Helicities: Arnold-Beltrami-Childress fields (curl f = k f) give H_kin = k E_kin and
+  H_mag = E_mag / k, which fixes the signs of Curl and VectorPotential and the dV factor.
 * LSRK54: order conditions of the stability polynomial (Taylor series through z^4).
 * Alias index table: the kept ranges listed for the FourierFlows 1/3 rule.
 

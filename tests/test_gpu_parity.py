@@ -69,10 +69,14 @@ def _resample(ms, nx, ny, nz, B):
 @pytest.mark.parametrize("n", [8, 16, 24, 32, 48, 64, 96])
 @pytest.mark.parametrize("T,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
 def test_initial_condition_r2c(n, T, tol):
-    say("ic r2c", n, T.__name__)
-    o, p = pair(n, T=T)
+    # Float64 MHD at nx = 24 is refused (its row kernel is disabled): that case transforms
+    # the three velocity fields of an HD problem instead
+    physics = "HD" if (n == 24 and T == np.float64) else "MHD"
+    say("ic r2c", n, T.__name__, physics)
+    o, p = pair(n, T=T, physics=physics)
     assert rel_l2(p.sol, o.sol) < tol
-    assert rel_l2(p.vars.bx, o.vars[3]) == 0.0
+    last = 3 if physics == "MHD" else 0
+    assert rel_l2(p.vars.field(last), o.vars[last]) == 0.0
 
 
 @pytest.mark.parametrize("shape", [(32, 16, 24), (16, 48, 32), (64, 32, 16)])
@@ -90,7 +94,8 @@ def _calcN_gpu(p, sol_host):
 
 
 FAULTED_24_MHD_FP64 = ("24^3 MHD Float64 row kernel faulted the GPU once (memory aperture violation) in "
-                       "a spill-heavy build; no cause was shown, so this case stays off the GPU")
+                       "a spill-heavy build; no cause was shown, so this case stays off the GPU (the "
+                       "library refuses the configuration)")
 
 
 @pytest.mark.parametrize("physics", ["MHD", "HD"])

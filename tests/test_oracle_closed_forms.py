@@ -188,3 +188,81 @@ def test_cfl_and_integrator_step_count():
     assert cfl_dt(p, td, 0.3) == np.float32(min(0.3 * dl / vmax, td))
     time_integrator(p, 1e9, 4, usr_dt=1e-3)
     assert p.step == 5  # N0+1 steps (integrator.jl:76,104)
+
+
+@pytest.mark.parametrize("mode", ["literal", "fused"])
+def test_hd_equals_mhd_velocity_with_zero_b(mode):
+    """With b = 0 the MHD velocity right-hand side is -P(i k.(u u)) - nu k^2 u, the HD one
+    (HDSolver.jl:58-74 vs MHDSolver.jl:63-94); the MHD sign of the u u term is fixed by
+    cross-helicity conservation and the Alfven wave, so this pins the HD sign."""
+    n = 16
+    ms = ic.solenoidal_modes(9, k0=2.0, kmax=6.0, mhd=False)
+    u = ms.real_fields(n, nf=3, dtype=np.float64)
+    zero = np.zeros_like(u[0])
+    hd = OracleProblem(nx=n, nu=0.02, B_field=False, T=np.float64, mode=mode)
+    hd.set_ic(*u)
+    mh = OracleProblem(nx=n, nu=0.02, eta=0.05, B_field=True, T=np.float64, mode=mode)
+    mh.set_ic(*u, zero, zero, zero)
+    Nh = np.zeros_like(hd.sol)
+    Nm = np.zeros_like(mh.sol)
+    calcN(Nh, hd.sol, 0.0, hd)
+    calcN(Nm, mh.sol, 0.0, mh)
+    assert rel_l2(Nh, Nm[0:3]) < 1e-14
+    assert np.max(np.abs(Nm[3:6])) == 0.0
+    assert np.max(np.abs(Nh)) > 1.0  # a non-trivial right-hand side
+
+
+@pytest.mark.parametrize("physics", ["HD", "MHD"])
+@pytest.mark.parametrize("mode", ["literal", "fused"])
+def test_taylor_green_analytic_rhs(physics, mode):
+    """u = (sin x cos y cos z, -cos x sin y cos z, 0), b = 0: the incompressible right-hand side
+    is -(u.grad)u - grad p - nu k^2 u with p = (cos 2x + cos 2y)(2 + cos 2z)/16, i.e.
+    N = (-sin 2x cos 2z / 8, -sin 2y cos 2z / 8, (cos 2x + cos 2y) sin 2z / 8) - 3 nu u."""
+    n, nu = 32, 0.01
+    p = OracleProblem(nx=n, nu=nu, eta=0.03, B_field=(physics == "MHD"), T=np.float64, mode=mode)
+    u, _ = ic.taylor_green(n, dtype=np.float64)
+    zero = np.zeros_like(u[0])
+    p.set_ic(*u, *([zero] * 3 if physics == "MHD" else []))
+    N = np.zeros_like(p.sol)
+    calcN(N, p.sol, 0.0, p)
+    X, Y, Z = ic.grid_coords(n)
+    s, c = np.sin, np.cos
+    expect = [-s(2 * X) * c(2 * Z) / 8, -s(2 * Y) * c(2 * Z) / 8, (c(2 * X) + c(2 * Y)) * s(2 * Z) / 8]
+    for a in range(3):
+        got = np.fft.irfftn(N[a], s=(n, n, n), axes=(0, 1, 2))
+        assert np.max(np.abs(got - (expect[a] - 3 * nu * u[a]))) < 1e-13, a
+    if physics == "MHD":
+        assert np.max(np.abs(N[3:6])) == 0.0
+
+
+def _abc(k, A, B, C, n):
+    """Arnold-Beltrami-Childress field of wavenumber k: curl f = k f."""
+    X, Y, Z = ic.grid_coords(n)
+    s, c = np.sin, np.cos
+    return [A * s(k * Z) + C * c(k * Y), B * s(k * X) + A * c(k * Z), C * s(k * Y) + B * c(k * X)]
+
+
+def test_helicities_beltrami_closed_forms():
+    """For curl u = k_u u: H_kin = sum u.(curl u) dV = k_u E_kin.  For curl b = k_b b the
+    Coulomb-gauge potential is A = b / k_b, so H_mag = sum A.b dV = E_mag / k_b.  The signs of
+    Curl (VectorCalculus.jl:24-51) and VectorPotential (MHDAnalysis.jl:138-174) and the dV
+    of the volume integrals are all fixed by these."""
+    n, ku, kb = 24, 2, 3
+    u = _abc(ku, 0.7, -0.4, 1.1, n)
+    b = _abc(kb, 0.3, 0.9, -0.5, n)
+    p = OracleProblem(nx=n, B_field=True, T=np.float64)
+    p.set_ic(*u, *b)
+    d = full_diagnostics(p)
+    assert math.isclose(d["H_kin"], ku * d["E_kin"], rel_tol=1e-12)
+    assert math.isclose(d["H_mag"], d["E_mag"] / kb, rel_tol=1e-12)
+    dV = (2 * math.pi / n) ** 3
+    assert math.isclose(d["E_kin"], dV * float(np.sum(np.square(u))), rel_tol=1e-14)
+    # a left-handed b (curl b = -k b) flips the sign of H_mag
+    p2 = OracleProblem(nx=n, B_field=True, T=np.float64)
+    X, Y, Z = ic.grid_coords(n)
+    ul = [np.sin(ku * Z), np.cos(ku * Z), np.zeros_like(Z)]      # curl = +k u
+    bl = [np.cos(kb * Z), np.sin(kb * Z), np.zeros_like(Z)]      # curl = -k b
+    p2.set_ic(*ul, *bl)
+    d2 = full_diagnostics(p2)
+    assert math.isclose(d2["H_kin"], ku * d2["E_kin"], rel_tol=1e-12)
+    assert math.isclose(d2["H_mag"], -d2["E_mag"] / kb, rel_tol=1e-12)
