@@ -113,6 +113,15 @@ int32_t mhd_set_sol(mhd_handle* h, const void* host_in);
  * array indices per entry, values holds nf complex T per entry (field-fastest).
  * vars are refreshed from the dealiased state (a spectral SetUpProblemIC!). */
 int32_t mhd_set_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values);
+/* mhd_set_sol_modes that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops admitting requests,
+ * code that forbids hipMalloc).  The staging of idx and values lives in a library-owned
+ * 4 MiB device array that each GPU holds until the process ends, so nmodes is limited to
+ * what fits (about 38800 Float64 MHD entries, 69900 Float32 MHD entries); larger calls
+ * fail with MHD_ERR_INVALID.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (from several threads or handles) race.  sol is zeroed by a fill kernel, not
+ * a memset.  Same result as mhd_set_sol_modes. */
+int32_t mhd_set_sol_modes_noalloc(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values);
 /* Read nmodes entries of prob.sol: idx holds (kx, l, m) array indices per entry, values_out
  * receives nf complex T per entry (field-fastest).  A partial read of the state (no
  * counterpart in the reference, which copies whole arrays), e.g. the modes a coarser grid

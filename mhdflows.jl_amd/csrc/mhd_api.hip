@@ -26,6 +26,35 @@ struct mhd_handle {
   EngineBase* e;
 };
 
+// Device staging of mhd_set_sol_modes_noalloc: the (kx, l, m) triples and the values of
+// one call.  Every GPU the library runs on holds its own copy until the process ends; all
+// calls on one GPU share it, so such calls must not overlap on that GPU.
+constexpr size_t kScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// Device address of `scratch` per device ordinal.  The symbol's host-side name is not a
+// device address; this one, taken once per device with that device current, is.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+
+int scratch_ptr(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  *out = p;
+  return 0;
+}
+}  // namespace
+
 struct mhd_batch {
   struct Group {
     EngineBase* e;
@@ -155,6 +184,23 @@ int32_t mhd_batch_set_sol_modes(mhd_batch* b, int32_t run, int64_t nmodes, const
   BATCH_RUN(b, run);
   if (nmodes > 0 && (!idx || !values)) return fail(E_INVALID, "idx and values are required");
   return G_.e->set_sol_modes(L_, nmodes, idx, values);
+}
+int32_t mhd_set_sol_modes_noalloc(mhd_handle* h, int64_t nmodes, const int32_t* idx, const void* values) {
+  MHD_TRY(check_handle(h));  // makes the handle's device current
+  if (nmodes < 0) return fail(E_INVALID, "nmodes must be >= 0");
+  if (nmodes > 0 && (!idx || !values)) return fail(E_INVALID, "idx and values are required");
+  const size_t need = h->e->mode_stage_bytes(nmodes);
+  if (need > kScratchBytes) {
+    int64_t most = 0;
+    while (h->e->mode_stage_bytes(most + 1024) <= kScratchBytes) most += 1024;
+    while (h->e->mode_stage_bytes(most + 1) <= kScratchBytes) ++most;
+    return fail(E_INVALID, std::to_string(nmodes) + " modes need " + std::to_string(need) +
+                               " bytes of staging, more than the 4 MiB scratch (at most " + std::to_string(most) +
+                               " modes for this precision and physics); use mhd_set_sol_modes");
+  }
+  void* p = nullptr;
+  MHD_TRY(scratch_ptr(h->cfg.device, &p));
+  return h->e->set_sol_modes_staged(0, nmodes, idx, values, p, kScratchBytes);
 }
 int32_t mhd_get_sol_modes(mhd_handle* h, int64_t nmodes, const int32_t* idx, void* values_out) {
   MHD_TRY(check_handle(h));

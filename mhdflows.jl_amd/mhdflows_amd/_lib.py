@@ -61,6 +61,7 @@ SIGNATURES = {
     "mhd_set_sol_modes": (_I, [_P, C.c_int64, _P, _P]),
     "mhd_batch_set_sol_modes": (_I, [_P, _I, C.c_int64, _P, _P]),
     "mhd_get_sol_modes": (_I, [_P, C.c_int64, _P, _P]),
+    "mhd_set_sol_modes_noalloc": (_I, [_P, C.c_int64, _P, _P]),
     "mhd_batch_get_sol_modes": (_I, [_P, _I, C.c_int64, _P, _P]),
     "mhd_get_vars": (_I, [_P, _I, _P]),
     "mhd_device_ptrs": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
