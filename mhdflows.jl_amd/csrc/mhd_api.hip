@@ -89,7 +89,9 @@ int validate(const mhd_config* c) {
   return 0;
 }
 
-Geom geom_of(const mhd_config* c) { return make_geom(c->nx, c->ny, c->nz, c->Lx, c->Ly, c->Lz); }
+Geom geom_of(const mhd_config* c) {
+  return make_geom(c->nx, c->ny, c->nz, c->Lx, c->Ly, c->Lz, c->precision == MHD_F64 ? 16 : 8);
+}
 
 RunParams params_of(const mhd_config* c) {
   RunParams p;

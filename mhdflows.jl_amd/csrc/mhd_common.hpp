@@ -31,14 +31,22 @@ struct Geom {
   // The work buffer's z-planes are padded: planes a multiple of 8 KB apart put the nz
   // elements of a z column on few DRAM channels, and the z passes, which read and write
   // such columns, then run well below the copy rate.
-  int64_t wps;     // work buffer plane stride (complex elements) = ny*nkr + WORK_PLANE_PAD
+  // The work buffer's rows start on WORK_ROW_ALIGN-byte boundaries: with nkr-long rows
+  // (e.g. 129 complex) the strips of columns that the y and z passes move straddle an extra
+  // cache line in most rows, and the column access patterns run 30-50 % slower.
+  int wrs;         // work buffer row stride (complex elements) >= nkr
+  int64_t wps;     // work buffer plane stride (complex elements) = ny*wrs + WORK_PLANE_PAD
   int64_t wnc;     // work buffer field stride = nz*wps
 };
 
-constexpr int WORK_PLANE_PAD = 32;  // complex elements added to every z-plane of the work buffer
+constexpr int WORK_PLANE_PAD = 32;     // complex elements added to every z-plane of the work buffer
+// Measured on MI355X at 256^3 Float32: 129-complex rows 288.6 steps/s, 64 B aligned rows
+// 306.0, 128 B aligned rows 318.6 (column passes 12-21 % faster).
+constexpr int WORK_ROW_ALIGN = 128;  // bytes
 
-// Plane and field strides of a complex buffer (only the work buffer is padded) and the
+// Row, plane and field strides of a complex buffer (only the work buffer is padded) and the
 // offset of row (z, y) = row index z*ny + y in it.
+__host__ __device__ inline int64_t row_stride(const Geom& g, int sel) { return sel == B_WORK ? g.wrs : g.nkr; }
 __host__ __device__ inline int64_t plane_stride(const Geom& g, int sel) {
   return sel == B_WORK ? g.wps : (int64_t)g.ny * g.nkr;
 }
@@ -47,7 +55,7 @@ __host__ __device__ inline int64_t row_offset(const Geom& g, int sel, int64_t ro
   if (sel != B_WORK) return row * g.nkr;
   const int r = (int)row;  // rows of one run: ny*nz < 2^31
   const int z = r / g.ny;
-  return (int64_t)z * g.wps + (int64_t)(r - z * g.ny) * g.nkr;
+  return (int64_t)z * g.wps + (int64_t)(r - z * g.ny) * g.wrs;
 }
 
 // Division by an invariant integer d via multiply and shift (l = ceil(log2 d),
@@ -75,7 +83,8 @@ inline void aliased_block(int n, int* lo, int* hi) {
   *hi = iR;
 }
 
-inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
+// cx_bytes: bytes of one complex value of the run's precision (8 or 16)
+inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz, int cx_bytes) {
   Geom g;
   g.nx = nx; g.ny = ny; g.nz = nz; g.nkr = nx / 2 + 1;
   int lo, hi;
@@ -86,7 +95,9 @@ inline Geom make_geom(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
   aliased_block(nz, &lo, &hi);
   g.mz_lo_end = lo; g.mz_hi_begin = hi;
   g.nc = (int64_t)nz * ny * g.nkr;
-  g.wps = (int64_t)ny * g.nkr + WORK_PLANE_PAD;
+  const int ra = WORK_ROW_ALIGN / cx_bytes > 1 ? WORK_ROW_ALIGN / cx_bytes : 1;
+  g.wrs = (g.nkr + ra - 1) / ra * ra;
+  g.wps = (int64_t)ny * g.wrs + WORK_PLANE_PAD;
   g.wnc = (int64_t)nz * g.wps;
   g.nr = (int64_t)nz * ny * nx;
   const double twopi = 6.283185307179586;  // Julia's 2π as Float64

@@ -677,10 +677,10 @@ class Engine : public EngineBase {
     MHD_TRY(r2c_vars_to_work(run));
     for (int f = 0; f < nf; ++f) {
       if (!fields[f]) continue;
-      // work planes are padded: copy plane by plane into the unpadded state layout
-      const size_t pl = sizeof(cx<T>) * (size_t)g.ny * g.nkr;
-      MHD_HIP(hipMemcpy2DAsync(b.buf[B_SOL] + (int64_t)f * g.nc, pl, b.buf[B_WORK] + (int64_t)f * g.wnc,
-                               sizeof(cx<T>) * (size_t)g.wps, pl, (size_t)g.nz, hipMemcpyDeviceToDevice, st));
+      // work rows and planes are padded: copy into the unpadded state layout
+      MHD_TRY(timed("unpad", 0, [&] {
+        return launch_unpad<T>(g, b.buf[B_WORK] + (int64_t)f * g.wnc, b.buf[B_SOL] + (int64_t)f * g.nc, st);
+      }));
     }
     MHD_TRY(vars_maxima());
     MHD_HIP(hipStreamSynchronize(st));

@@ -63,10 +63,11 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const Geom& G = a.g;
   // element (along the axis) and other-index strides of the source and the destination
   const int64_t sps = plane_stride(G, a.src), dps = plane_stride(G, a.dst);
-  const int64_t s_estride = a.axis == 1 ? (int64_t)G.nkr : sps;
-  const int64_t s_ostride = a.axis == 1 ? sps : (int64_t)G.nkr;
-  const int64_t d_estride = a.axis == 1 ? (int64_t)G.nkr : dps;
-  const int64_t d_ostride = a.axis == 1 ? dps : (int64_t)G.nkr;
+  const int64_t srs = row_stride(G, a.src), drs = row_stride(G, a.dst);
+  const int64_t s_estride = a.axis == 1 ? srs : sps;
+  const int64_t s_ostride = a.axis == 1 ? sps : srs;
+  const int64_t d_estride = a.axis == 1 ? drs : dps;
+  const int64_t d_ostride = a.axis == 1 ? dps : drs;
   auto ldsf = [&](int pos) -> cx<T>& { return lds[pos * S + c]; };
 
   const int64_t nb = (int64_t)gridDim.x;
@@ -664,7 +665,7 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
       for (int f = 0; f < NF; ++f) sH[f] = s[j][f];
     }
     cx<T> R[NF];
-    const int64_t qp = q + (plane_stride(G, a.prod_sel) - (int64_t)G.ny * G.nkr) * ms[j];
+    const int64_t qp = ms[j] * plane_stride(G, a.prod_sel) + ls[j] * row_stride(G, a.prod_sel) + kx;
     rhs_at<T, PHYS>(G, rb, a, qp, k, sH, R);
     if (PHYS == PHYS_MHD && a.add_force && rb.force) {
       const int K = rb.force_K;
@@ -934,7 +935,7 @@ diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ 
   const int kx = (int)(i % G.nkr);
   const int64_t rest = i / G.nkr;
   const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
-  const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.nkr + kx;
+  const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.wrs + kx;
   const double k0 = kx * G.multx, k1 = signed_index(l, G.ny) * G.multy, k2 = signed_index(m, G.nz) * G.multz;
   const cx<T> a0 = src[idx], a1 = src[G.wnc + idx], a2 = src[2 * G.wnc + idx];
   auto ik = [](double kk, cx<T> z) { return cx<T>{T(-kk * (double)z.im), T(kk * (double)z.re)}; };
@@ -979,6 +980,18 @@ gather_modes_kernel(Geom G, const cx<T>* __restrict__ x, int nf, int64_t nmodes,
   const bool ok = kx >= 0 && kx < G.nkr && l >= 0 && l < G.ny && m >= 0 && m < G.nz;
   const int64_t q = ok ? ((int64_t)m * G.ny + l) * G.nkr + kx : 0;
   for (int f = 0; f < nf; ++f) vals[i * nf + f] = ok ? x[(int64_t)f * G.nc + q] : cx<T>{T(0), T(0)};
+}
+
+// copy one field of the work buffer (padded rows and planes) into the unpadded state layout
+template <class T>
+__global__ void __launch_bounds__(256)
+unpad_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= G.nc) return;
+  const int kx = (int)(i % G.nkr);
+  const int64_t rest = i / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  dst[i] = src[(int64_t)m * G.wps + (int64_t)l * G.wrs + kx];
 }
 
 // zero n complex values (a fill kernel: usable where a memset node is not wanted)

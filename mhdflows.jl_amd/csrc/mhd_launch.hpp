@@ -87,6 +87,8 @@ hipError_t launch_gather(const Geom& g, const cx<T>* x, int nf, int64_t nmodes, 
                          hipStream_t s);
 template <class T>
 hipError_t launch_zero(cx<T>* x, int64_t n, hipStream_t s);
+template <class T>
+hipError_t launch_unpad(const Geom& g, const cx<T>* work_field, cx<T>* dst, hipStream_t s);
 
 enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3, SK_PRUNE = 4 };
 
