@@ -180,9 +180,32 @@ int32_t mhd_set_step_pruning(mhd_handle* h, int32_t enable);
 int32_t mhd_diagnostics(mhd_handle* h, mhd_diag* out);
 /* Refresh vars from sol (C2R of the dealiased state) without stepping. */
 int32_t mhd_refresh_vars(mhd_handle* h);
-/* calcF! hook (src/pgen.jl:231-234): seeded time-independent solenoidal forcing on
- * the shell round(|k|) == kf with amplitude amp (seed 0 and amp 0 turn it off). */
+/* calcF! hook (src/pgen.jl:231-234).  A problem has one calcF!: each call below replaces the
+ * forcing set before; forcing acts on the MHD right-hand side after the advection and is
+ * discarded for HD, as in the reference (src/pgen.jl:173-181 adds it before
+ * HDcalcN_advection! zeroes N).  Module buffers are allocated when a module is set.
+ *
+ * Seeded time-independent solenoidal forcing on the shell round(|k|) == kf with amplitude
+ * amp (the large-scale forcing of the 1024^3 precision run; amp = 0 turns every forcing off). */
 int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp);
+/* N97ForceDriving! (src/pgen/TaylorGreenDynamo.jl:12-16) with SetUpN97!(prob; F0, kf) (:18-33),
+ * or any static forcing field: f3 holds 3 real fields (nx*ny*nz values of T each, NULL = zero;
+ * for N97 the caller evaluates F0 sin(kf x) cos(kf y) cos(kf z) and -F0 cos(kf x) sin(kf y)
+ * cos(kf z) on the grid in T, as SetUpN97! does), transformed on the device and added to the
+ * velocity right-hand side, N_u += R2C(f).  Allocates 3 complex fields. */
+int32_t mhd_set_forcing_field(mhd_handle* h, const void* const* f3);
+/* NDForceDriving! (src/pgen/NegativeDamping.jl:23-44) with SetUpND!(prob, P, fx, fy, fz)
+ * (:16-22): every right-hand side adds A R2C(f_i u_i) to N_u_i, A = P / (sum|u_i^2 f_i| dV)
+ * with u the real fields of the stage input; the sum is accumulated in Float64 (the
+ * reference sums in T).  f3: 3 real fields (NULL = zero).  Single-run problems only. */
+int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3);
+/* A99ForceDriving! (src/pgen/A99ForceDriving.jl:33-60) with SetUpFk(prob; kf, P, σ²)
+ * (:93-127), usr_vars.b = b and usr_vars.A = A.  The reference draws unseeded rand() per
+ * mode and call; here the four draws of a mode (phase 1, the complex Φ, phase 2) are
+ * counter-based: splitmix64(seed·0x9E3779B97F4A7C15 + (counter << 32) + 1 + mode index),
+ * counter = 4·c + draw, c = 8·clock.step + stage inside a step and 2^40 + n for the n-th
+ * mhd_calcN call of the handle; mode index = (m·ny + l)·nkr + kx. */
+int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, double sigma2, double b, double A);
 
 /* ---- measurement -------------------------------------------------------------- */
 /* Autotuner: time every compiled variant of the hot launches (column-pass strip

@@ -299,6 +299,20 @@ int32_t mhd_set_forcing(mhd_handle* h, uint64_t seed, double kf, double amp) {
   return h->e->set_forcing(0, seed, kf, amp);
 }
 
+int32_t mhd_set_forcing_field(mhd_handle* h, const void* const* f3) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_forcing_field(0, f3);
+}
+int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3) {
+  MHD_TRY(check_handle(h));
+  if (!f3) return fail(E_INVALID, "f3 is NULL");
+  return h->e->set_forcing_nd(0, P, f3);
+}
+int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, double sigma2, double b, double A) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_forcing_a99(0, seed, kf, P, sigma2, b, A);
+}
+
 int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len) {
   MHD_TRY(check_handle(h));
   return h->e->tune(reps, report, report_len);

@@ -131,6 +131,7 @@ struct RunClock {
   int32_t nan_flag;  // set when the maxima are NaN
   int64_t target;    // step launches skip this run once step >= target (pool slots)
   int64_t prune_until;  // steps ending before this step count skip the aliased modes (see step_pruned)
+  double nd_amp;     // negative damping: A = P / (sum |u_i^2 f_i| dV) of the current right-hand side
 };
 
 // Step launches of a run whose clock reached its target do nothing (finished or
@@ -163,6 +164,15 @@ struct RunBuf {
   int force_K;
   int nnu, neta;
   double nu, eta;
+  // calcF! modules of the reference (at most one is set):
+  const cx<T>* fdense;  // static forcing spectrum added to N_u (3 fields of nc; N97, user fields) or null
+  const T* ndf;         // negative damping: the real fields f_x, f_y, f_z (nr each) or null
+  double* ndpart;       // negative damping: per row-block FP64 sums of |u_i^2 f_i| (row kernel output)
+  double nd_P;          // negative damping: injected power P
+  int a99;              // A99 stochastic forcing on
+  uint64_t a99_seed;
+  double a99_kf, a99_s2, a99_b, a99_A, a99_amp;  // kf, sigma^2, b, usr A, setup amplitude (all rounded to T)
+  double a99_den;       // tanh(b pi / 2) evaluated in T
 };
 
 }  // namespace mhd

@@ -105,6 +105,11 @@ class Engine : public EngineBase {
   };
   std::vector<Ev> evs;
   std::vector<void*> force_tabs;
+  // calcF! module buffers per run: static forcing spectrum, negative-damping f fields and sums
+  std::vector<void*> fdense_bufs, ndf_bufs, ndpart_bufs;
+  bool nd_any = false;     // a run has negative damping: MHD row launches form 12 products
+  int np_alloc = 0;        // product fields each run's work buffer holds
+  int64_t calls_ = 0;      // right-hand sides evaluated by calcN (draw counter of stochastic forcing)
   double* dpart = nullptr;     // FP64 diagnostic partials
   int dpart_blocks = 0;
 
@@ -115,6 +120,9 @@ class Engine : public EngineBase {
     for (void* p : allocs) hipFree(p);
     for (void* p : force_tabs)
       if (p) hipFree(p);
+    for (auto* v : {&fdense_bufs, &ndf_bufs, &ndpart_bufs})
+      for (void* p : *v)
+        if (p) hipFree(p);
     if (dpart) hipFree(dpart);
     if (stage_) hipFree(stage_);
     if (rbd) hipFree(rbd);
@@ -162,6 +170,10 @@ class Engine : public EngineBase {
     apply_cached_tuning();
     rbh.resize(nr_);
     force_tabs.assign(nr_, nullptr);
+    fdense_bufs.assign(nr_, nullptr);
+    ndf_bufs.assign(nr_, nullptr);
+    ndpart_bufs.assign(nr_, nullptr);
+    np_alloc = np;
     for (int r = 0; r < nr_; ++r) {
       RunBuf<T>& b = rbh[r];
       std::memset(&b, 0, sizeof(b));
@@ -520,7 +532,11 @@ class Engine : public EngineBase {
     const bool fused_planes = prune && sp.mask_input && sp.self_sel < 0;
     MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune, true, fused_planes));
     MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune, true));
-    RowArgs ra = row_args(row_var);
+    // negative damping adds three products (f_i u_i) and its sum to the MHD row launch
+    const bool nd = nd_any && phys == PHYS_MHD && sp.add_force;
+    const int rvar = nd ? 0 : row_var;
+    const int npf = nd ? 12 : np;
+    RowArgs ra = row_args(rvar);
     ra.in_sel = B_WORK;
     ra.in_ncols = prune ? g.kx_keep : g.nkr;
     ra.out_sel = B_WORK;
@@ -530,11 +546,15 @@ class Engine : public EngineBase {
     ra.prune_out = gate_;
     const int kout = host_pruned_ ? g.kx_keep : g.nkr;
     // the real fields are stored at the last stage of a full step only
-    double bytes = rows * (nf * ra.in_ncols + np * kout) * sizeof(cx<T>) +
-                   (last && !host_pruned_ ? rows * nf * g.nx * sizeof(T) : 0);
-    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? ROW_MHD : ROW_HD, nf, bytes, row_var));
-    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));
-    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, np, false, true, false, gate_ != 0));
+    double bytes = rows * (nf * ra.in_ncols + npf * kout) * sizeof(cx<T>) +
+                   (last && !host_pruned_ ? rows * nf * g.nx * sizeof(T) : 0) + (nd ? rows * 3 * g.nx * sizeof(T) : 0);
+    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? (nd ? ROW_MHD_ND : ROW_MHD) : ROW_HD, nf, bytes, rvar));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, npf, false, true, false, gate_ != 0));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, npf, false, true, false, gate_ != 0));
+    if (nd) {
+      const double dV = (double)clk_dx(0) * (double)clk_dx(1) * (double)clk_dx(2);
+      MHD_TRY(timed("nd_amp", 0, [&] { return launch_nd_amp<T>(rbd, nr_, ra.blocks_per_run, dV, gate_, st); }));
+    }
     const int64_t used_blocks = ra.blocks_per_run;
     sp.g = g;
     sp.nruns = nr_;
@@ -553,7 +573,7 @@ class Engine : public EngineBase {
     const double c = sizeof(cx<T>) * (double)nr_;
     const double kept = (double)g.kx_keep * nkept_y(g) * nkept_z(g);
     const double all = (sp.gate && host_pruned_) ? kept : (double)g.nc;  // pruned step: kept modes only
-    double rd = np * all, wr = 0;
+    double rd = (np + forcing_fields()) * all, wr = 0;
     if (sp.mode == SP_RK4) {
       rd += nf * kept;                                  // stage input
       if (sp.stage > 0) rd += nf * kept + nf * all;     // base sol + acc
@@ -568,9 +588,24 @@ class Engine : public EngineBase {
     return (rd + wr) * c;
   }
 
+  // extra complex fields the spectral launch reads per mode for forcing (dense spectrum or
+  // negative-damping products)
+  int forcing_fields() const {
+    for (int r = 0; r < nr_; ++r)
+      if (rbh[r].fdense || rbh[r].ndf) return 3;
+    return 0;
+  }
+
+  bool any_module() const {
+    for (int r = 0; r < nr_; ++r)
+      if (rbh[r].fdense || rbh[r].ndf || rbh[r].a99) return true;
+    return false;
+  }
+
   SpecArgs spec_base() const {
     SpecArgs s{};
     s.g = g;
+    s.modules = any_module() ? 1 : 0;
     s.mask_input = 1;
     s.add_force = 1;
     s.self_sel = -1;
@@ -904,6 +939,7 @@ class Engine : public EngineBase {
     sp.mask_input = advection_only ? 0 : 1;
     sp.zero_aliased = advection_only ? 0 : 1;
     sp.add_force = advection_only ? 0 : 1;
+    if (!advection_only) sp.draw_ctr = (int64_t(1) << 40) + calls_++;
     MHD_TRY(rhs(B_EXT_SOL, !advection_only, sp, true));
     MHD_HIP(hipStreamSynchronize(st));
     return 0;
@@ -1120,15 +1156,150 @@ class Engine : public EngineBase {
   }
 
   // ---- forcing -------------------------------------------------------------------
+  // Drop every calcF! module of a run (the reference holds one calcF! per problem, so setting
+  // a module replaces the previous one).  The captured step is rebuilt on the next step.
+  int clear_forcing(int run) {
+    MHD_HIP(hipStreamSynchronize(st));
+    for (auto* v : {&force_tabs, &fdense_bufs, &ndf_bufs, &ndpart_bufs}) {
+      if ((*v)[run]) (void)hipFree((*v)[run]);
+      (*v)[run] = nullptr;
+    }
+    RunBuf<T>& b = rbh[run];
+    b.force = nullptr;
+    b.force_K = 0;
+    b.fdense = nullptr;
+    b.ndf = nullptr;
+    b.ndpart = nullptr;
+    b.nd_P = 0;
+    b.a99 = 0;
+    nd_any = false;
+    for (int r = 0; r < nr_; ++r) nd_any = nd_any || rbh[r].ndf != nullptr;
+    drop_graph();
+    return 0;
+  }
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+  int forcing_done() {
+    MHD_TRY(upload_runs());
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  int device_alloc(size_t bytes, void** p) {
+    *p = nullptr;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      return fail(E_HIP, "hipMalloc failed (out of device memory?)");
+    }
+    return 0;
+  }
+
+  // N97ForceDriving! / a static forcing field: R2C of the real fields f3 (NULL = zero) on the
+  // device (SetUpN97!, src/pgen/TaylorGreenDynamo.jl:29-32), added to N_u of every MHD call
+  int set_forcing_field(int run, const void* const* f3) override {
+    MHD_TRY(check_run(run));
+    MHD_TRY(clear_forcing(run));
+    RunBuf<T>& b = rbh[run];
+    for (int c = 0; c < 3; ++c) {
+      if (f3 && f3[c]) MHD_HIP(hipMemcpyAsync(b.rscr + (int64_t)c * g.nr, f3[c], sizeof(T) * g.nr, hipMemcpyHostToDevice, st));
+      else MHD_HIP(hipMemsetAsync(b.rscr + (int64_t)c * g.nr, 0, sizeof(T) * g.nr, st));
+    }
+    {
+      RunScope scope(this, run);
+      RowArgs ra = row_args();
+      ra.in_f0 = 0;
+      ra.real_sel = 1;
+      ra.out2_sel = B_WORK;
+      ra.out2_f0 = 0;
+      MHD_TRY(xrow("xrow_r2c", ra, ROW_R2C, 3, 0));
+      MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, 3, false));
+      MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    }
+    void* d = nullptr;
+    MHD_TRY(device_alloc(sizeof(cx<T>) * 3 * (size_t)g.nc, &d));
+    fdense_bufs[run] = d;
+    for (int c = 0; c < 3; ++c)
+      MHD_TRY(timed("unpad", 0, [&] {
+        return launch_unpad<T>(g, b.buf[B_WORK] + (int64_t)c * g.wnc, (cx<T>*)d + (int64_t)c * g.nc, st);
+      }));
+    b.fdense = (const cx<T>*)d;
+    return forcing_done();
+  }
+
+  // NDForceDriving! with SetUpND!(prob, P, fx, fy, fz) (src/pgen/NegativeDamping.jl:16-22)
+  int set_forcing_nd(int run, double P, const void* const* f3) override {
+    MHD_TRY(check_run(run));
+    if (nr_ != 1) return fail(E_INVALID, "negative damping acts on single-run problems");
+    MHD_TRY(clear_forcing(run));
+    RunBuf<T>& b = rbh[run];
+    void* d = nullptr;
+    MHD_TRY(device_alloc(sizeof(T) * 3 * (size_t)g.nr, &d));
+    ndf_bufs[run] = d;
+    for (int c = 0; c < 3; ++c) {
+      T* dst = (T*)d + (int64_t)c * g.nr;
+      if (f3 && f3[c]) MHD_HIP(hipMemcpyAsync(dst, f3[c], sizeof(T) * g.nr, hipMemcpyHostToDevice, st));
+      else MHD_HIP(hipMemsetAsync(dst, 0, sizeof(T) * g.nr, st));
+    }
+    MHD_TRY(device_alloc(sizeof(double) * (size_t)row_blocks, &d));
+    ndpart_bufs[run] = d;
+    if (phys == PHYS_MHD && np_alloc < 12) {  // room for the three f_i u_i products
+      MHD_HIP(hipStreamSynchronize(st));
+      void* w = nullptr;
+      MHD_TRY(device_alloc(sizeof(cx<T>) * (size_t)g.wnc * 12, &w));
+      for (auto& p : allocs)
+        if (p == (void*)b.buf[B_WORK]) p = w;
+      (void)hipFree(b.buf[B_WORK]);
+      b.buf[B_WORK] = (cx<T>*)w;
+      np_alloc = 12;
+    }
+    b.ndf = (const T*)ndf_bufs[run];
+    b.ndpart = (double*)ndpart_bufs[run];
+    b.nd_P = P;
+    nd_any = true;
+    return forcing_done();
+  }
+
+  // A99ForceDriving! with SetUpFk(prob; kf, P, sigma2) (src/pgen/A99ForceDriving.jl:93-127),
+  // usr_vars.b = b and usr_vars.A = A; evaluated in T as the reference's grid arrays are
+  int set_forcing_a99(int run, uint64_t seed, double kf, double P, double s2, double bb, double A) override {
+    MHD_TRY(check_run(run));
+    if (!(s2 > 0)) return fail(E_INVALID, "sigma^2 must be > 0");
+    MHD_TRY(clear_forcing(run));
+    const T kfT = (T)kf, s2T = (T)s2;
+    double integral = 0.0;  // sum(exp(-(k-kf)^2/s2) / (k+1)^2) over the half spectrum
+    for (int m = 0; m < g.nz; ++m)
+      for (int l = 0; l < g.ny; ++l)
+        for (int kx = 0; kx < g.nkr; ++kx) {
+          const T qx = (T)((double)kx * g.multx), qy = (T)((double)signed_index(l, g.ny) * g.multy);
+          const T qz = (T)((double)signed_index(m, g.nz) * g.multz);
+          const T K = (qx * qx + qy * qy) + qz * qz;
+          const T k = std::sqrt(K), d = k - kfT;
+          const T e = std::exp(-(d * d) / s2T);
+          const T k1 = k + T(1);
+          integral += (double)(T)(e * (T(1) / (k1 * k1)));
+        }
+    const double dx = (double)clk_dx(0), dy = (double)clk_dx(1), dz = (double)clk_dx(2);
+    const double amp =
+        std::sqrt(P * 3 * (lens[0] / dx) * (lens[1] / dy) * (lens[2] / dz) / integral * (1 / dx / dy / dz));
+    RunBuf<T>& b = rbh[run];
+    b.a99 = 1;
+    b.a99_seed = seed;
+    b.a99_kf = (double)kfT;
+    b.a99_s2 = (double)s2T;
+    b.a99_b = (double)(T)bb;
+    b.a99_A = (double)(T)A;
+    b.a99_amp = (double)(T)amp;
+    const T bT = (T)bb;
+    b.a99_den = (double)(T)std::tanh((T)((T)(bT * (T)3.14159265358979323846) / (T)2));
+    return forcing_done();
+  }
+
   int set_forcing(int run, uint64_t seed, double kf, double amp) override {
     MHD_TRY(check_run(run));
-    if (force_tabs[run]) {
-      MHD_HIP(hipStreamSynchronize(st));
-      hipFree(force_tabs[run]);
-      force_tabs[run] = nullptr;
-    }
-    rbh[run].force = nullptr;
-    rbh[run].force_K = 0;
+    MHD_TRY(clear_forcing(run));
     if (amp != 0.0 && kf > 0) {
       const int K = (int)std::ceil(kf + 0.5);
       const int64_t per = (int64_t)(K + 1) * (2 * K + 1) * (2 * K + 1);

@@ -135,6 +135,7 @@ enum RowKind {
   ROW_R2C = 3,        // NF real rows -> NF complex rows
   ROW_VARS_MHD = 4,   // 6 real rows -> 9 products and the 6 inputs -> r2c (15 rows)
   ROW_VARS_HD = 5,    // 3 real rows -> 6 products and the 3 inputs -> r2c (9 rows)
+  ROW_MHD_ND = 6,     // ROW_MHD plus negative damping: 3 more products f_i u_i and sum |u_i^2 f_i|
 };
 
 struct RowArgs {
@@ -165,13 +166,14 @@ template <int NF> struct RowKindInfo<ROW_C2R, NF> { static constexpr int NIN = N
 template <int NF> struct RowKindInfo<ROW_R2C, NF> { static constexpr int NIN = NF, NPROD = 0, NSELF = NF; };
 template <int NF> struct RowKindInfo<ROW_VARS_MHD, NF> { static constexpr int NIN = 6, NPROD = 9, NSELF = 6; };
 template <int NF> struct RowKindInfo<ROW_VARS_HD, NF> { static constexpr int NIN = 3, NPROD = 6, NSELF = 3; };
+template <int NF> struct RowKindInfo<ROW_MHD_ND, NF> { static constexpr int NIN = 6, NPROD = 12, NSELF = 0; };
 
 // Real-space products of one point; u = v[0..2], b = v[3..5].
 // MHD: T_ij = b_i b_j - u_i u_j (i<=j: 00,01,02,11,12,22), M_aj = u_a b_j - b_a u_j (01,02,12).
 // HD:  U_ij = u_i u_j.
 template <class T, int KIND>
 __device__ __forceinline__ T product(int p, const T* v) {
-  if constexpr (KIND == ROW_MHD || KIND == ROW_VARS_MHD) {
+  if constexpr (KIND == ROW_MHD || KIND == ROW_VARS_MHD || KIND == ROW_MHD_ND) {
     switch (p) {
       case 0: return v[3] * v[3] - v[0] * v[0];
       case 1: return v[3] * v[4] - v[0] * v[1];
@@ -264,7 +266,8 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   using Rfwd = typename RReverse<Rinv>::type;
   constexpr int R0 = RFirst<Rinv>::value;   // first inverse radix == last forward radix
   constexpr int RL = RLast<Rinv>::value;    // last inverse radix == first forward radix
-  constexpr bool C2R_IN = (KIND == ROW_MHD || KIND == ROW_HD || KIND == ROW_C2R);
+  constexpr bool C2R_IN = (KIND == ROW_MHD || KIND == ROW_HD || KIND == ROW_C2R || KIND == ROW_MHD_ND);
+  constexpr bool ND = KIND == ROW_MHD_ND;
   constexpr int NIN = RowKindInfo<KIND, NF>::NIN;
   constexpr int NPROD = RowKindInfo<KIND, NF>::NPROD;
 
@@ -299,6 +302,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   // v[f][e]: the real samples 2j and 2j+1 of field f at the thread's positions
   // j = tr + i*TX + r*(M/RL) (last inverse pass layout == first forward pass layout).
   cx<T> v[NIN][E];
+  cx<T> fd[ND ? 3 : 1][E];  // negative damping: f_x, f_y, f_z at the same samples
 
   if constexpr (C2R_IN) {
     const T scale = T(G.inv_n);
@@ -399,6 +403,34 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
         part[2 * blk + 1] = red[1][0];
       }
     }
+    if constexpr (ND) {
+      // NDForceDriving! (src/pgen/NegativeDamping.jl:23-44): f_i at the thread's samples and the
+      // block's share of sum |u_i^2 f_i|, accumulated in Float64 (one partial per row block)
+      __shared__ double nds[128];
+      double sum = 0.0;
+      const gptr<const T> fb = gmem<const T>(rb.ndf);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const gptr<const T> fr = fb + (int64_t)c * G.nr + row * NX;
+#pragma unroll
+        for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const int j = tr + i * TX + r * (M / RL);
+            const int e = i * RL + r;
+            fd[c][e] = row_ok ? ldg((gptr<const cx<T>>)(fr + 2 * j)) : cx<T>{T(0), T(0)};
+            const double ur = v[c][e].re, ui = v[c][e].im;
+            sum += fabs(ur * ur * (double)fd[c][e].re) + fabs(ui * ui * (double)fd[c][e].im);
+          }
+      }
+      nds[t] = sum;
+      __syncthreads();
+      for (int st = 64; st > 0; st >>= 1) {
+        if (t < st) nds[t] += nds[t + st];
+        __syncthreads();
+      }
+      if (t == 0) gmem(rb.ndpart)[blk] = nds[0];
+    }
   } else {
     // real input rows (vars), loaded straight into the forward pass-0 layout
 #pragma unroll
@@ -426,7 +458,15 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
 #pragma unroll UNR
     for (int p = 0; p < NOUT; ++p) {
       cx<T> z[E];
-      if (p < NPROD) {
+      if (ND && p >= 9) {
+        // F_i = f_i u_i (NegativeDamping.jl:39); c is matched against p so that the register
+        // arrays are indexed by constants only
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          if (c == p - 9)
+#pragma unroll
+            for (int e = 0; e < E; ++e) z[e] = cx<T>{v[c][e].re * fd[c][e].re, v[c][e].im * fd[c][e].im};
+      } else if (p < NPROD) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           T pr[NIN > 0 ? NIN : 1], pi[NIN > 0 ? NIN : 1];
@@ -476,6 +516,9 @@ struct SpecArgs {
   int planes;         // 0: interior modes 1 <= kx < nx/2; 1: the kx = 0 and kx = nx/2 planes;
                       // 2: every mode, kx = 0 mirrors read from rb.plane (masked stage input)
   int gate;           // 1 inside a time step: skip runs whose clock reached its target
+  int64_t draw_ctr;   // SP_RHS: draw counter of stochastic forcing (a step uses 8 step + stage)
+  int modules;        // 1: a run has a calcF! module (dense field, negative damping, A99): the
+                      // launch uses the kernel instance that evaluates them
 };
 
 template <class T>
@@ -554,7 +597,104 @@ __device__ __forceinline__ void rhs_at(const Geom& G, const RunBuf<T>& rb, const
   }
 }
 
-template <class T, int PHYS, int PLANES>
+// ---------------------------------------------------------------------------
+// calcF! modules evaluated per mode
+// ---------------------------------------------------------------------------
+// Counter-based uniform in [0, 1): splitmix64 of seed * 0x9E3779B97F4A7C15 + (counter << 32)
+// + index + 1, top 53 bits (Float64) or top 24 bits (Float32, as rand(Float32)).
+__device__ __forceinline__ uint64_t smix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+template <class T>
+__device__ __forceinline__ T counter_u01(uint64_t seed, uint64_t counter, uint64_t index) {
+  const uint64_t z = smix64(seed * 0x9E3779B97F4A7C15ull + (counter << 32) + 1 + index);
+  if constexpr (sizeof(T) == 8) return (T)((double)(z >> 11) * (1.0 / 9007199254740992.0));
+  else return (T)(float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+// tanh of a complex Float64 by Kahan's method (tan / sinh form, as Julia's Base and FreeBSD's
+// ctanh evaluate it): with t = tan(y), beta = 1 + t^2, s = sinh(x), rho = sqrt(1 + s^2),
+// tanh(x + iy) = (beta rho s + i t) / (1 + beta s^2); |x| beyond the overflow bound gives +-1.
+__device__ __forceinline__ void ctanh_d(double x, double y, double* re, double* im) {
+  if (4.0 * fabs(x) > 710.4758600739439) {  // asinh(prevfloat(floatmax(Float64)))
+    *re = copysign(1.0, x);
+    *im = copysign(0.0, y * (isfinite(y) ? sin(2.0 * fabs(y)) : 1.0));
+    return;
+  }
+  const double t = tan(y), beta = 1.0 + t * t, sh = sinh(x), rho = sqrt(1.0 + sh * sh);
+  if (isinf(t)) {
+    *re = rho / sh;
+    *im = 1.0 / t;
+    return;
+  }
+  const double den = 1.0 + beta * sh * sh;
+  *re = beta * rho * sh / den;
+  *im = t / den;
+}
+
+// principal square root of a complex T: rho = sqrt((|z| + |x|) / 2), then
+// (rho, y / (2 rho)) for x >= 0 and (|y| / (2 rho), copysign(rho, y)) for x < 0
+template <class T>
+__device__ __forceinline__ cx<T> csqrt_t(cx<T> z) {
+  if (z.re == T(0) && z.im == T(0)) return cx<T>{T(0), z.im};
+  const T rho = sqrt((hypot(z.re, z.im) + fabs(z.re)) * T(0.5));
+  const T eta = (z.im / rho) * T(0.5);
+  if (z.re >= T(0)) return cx<T>{rho, eta};
+  return cx<T>{fabs(eta), copysign(rho, z.im)};
+}
+
+// A99ForceDriving! (src/pgen/A99ForceDriving.jl:33-60) with SetUpFk (:93-127) at one mode:
+// Fk = amp sqrt(exp(-(k - kf)^2 / s2) / 2 / pi) / k (zero on the kx = 0 plane), the unit
+// vectors e1 = (ky, -kx, 0) / k_perp and e2 = (kx kz, ky kz, -k_perp^2) / (k_perp k), and four
+// draws per mode and call: theta1, Phi (complex: rand! fills both parts of the Complex{T}
+// buffer), theta2.  2pi and pi/2 are Float64 literals in the reference, so the phases and the
+// tanh argument are evaluated in Float64 and rounded to T.
+template <class T>
+__device__ __forceinline__ void a99_add(const RunBuf<T>& rb, int kx, const ModeK<T>& k, uint64_t ctr, uint64_t q,
+                                        cx<T>* R) {
+  if (kx == 0) return;  // Fk[1,:,:] = 0 (A99ForceDriving.jl:110)
+  const T kk = sqrt(k.K), kinv = sqrt(k.invK);
+  const T d = kk - T(rb.a99_kf);
+  const T g = exp(-(d * d) / T(rb.a99_s2));
+  if (!(g > T(0))) return;  // Fk = 0: nothing is added
+  const T Fk = T(rb.a99_amp) * sqrt(g / T(2) / T(3.14159265358979323846)) * kinv;
+  const T kp = sqrt(k.kx * k.kx + k.ky * k.ky);
+  const T e1x = k.ky / kp, e1y = -k.kx / kp;
+  const T e2x = k.kx * k.kz / kp * kinv, e2y = k.ky * k.kz / kp * kinv, e2z = -kp * kinv;
+  const uint64_t c4 = ctr << 2;
+  const T u0 = counter_u01<T>(rb.a99_seed, c4, q), u1 = counter_u01<T>(rb.a99_seed, c4 + 1, q);
+  const T u2 = counter_u01<T>(rb.a99_seed, c4 + 2, q), u3 = counter_u01<T>(rb.a99_seed, c4 + 3, q);
+  const double twopi = 6.283185307179586, halfpi = 1.5707963267948966;
+  double sn, cs;
+  sincos((double)u0 * twopi, &sn, &cs);
+  const cx<T> ph1{T(cs), T(sn)};
+  const T pif = T(3.14159265358979323846);
+  const cx<T> Phi{u1 * pif, u2 * pif};
+  const double b = (double)T(rb.a99_b);
+  double tr, ti;
+  ctanh_d(b * ((double)Phi.re - halfpi), b * (double)Phi.im, &tr, &ti);
+  const double den = rb.a99_den;
+  const cx<T> gi{T(-tr / den), T(-ti / den)};
+  const cx<T> gi2 = cmul(gi, gi);
+  const cx<T> gj = csqrt_t(cx<T>{T(1) - gi2.re, -gi2.im});
+  sincos((double)u3 * twopi, &sn, &cs);
+  const cx<T> ph2{T(cs), T(sn)};
+  const T AF = T(rb.a99_A) * Fk;
+  const cx<T> w1 = cmul(ph1 * AF, gi);
+  R[0] = R[0] + w1 * e1x;   // :48
+  R[1] = R[1] + w1 * e1y;   // :49
+  const cx<T> w2 = cmul(ph2 * AF, gj);
+  R[0] = R[0] + w2 * e2x;   // :54
+  R[1] = R[1] + w2 * e2y;   // :55
+  R[2] = R[2] + w2 * e2z;   // :56
+}
+
+// MODS = 1 adds the calcF! modules (their registers would otherwise cost the plain launch
+// occupancy: 61 -> 72 VGPRs for Float32 MHD)
+template <class T, int PHYS, int PLANES, int MODS>
 __global__ void __launch_bounds__(256)
 spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
@@ -667,14 +807,36 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
     cx<T> R[NF];
     const int64_t qp = ms[j] * plane_stride(G, a.prod_sel) + ls[j] * row_stride(G, a.prod_sel) + kx;
     rhs_at<T, PHYS>(G, rb, a, qp, k, sH, R);
-    if (PHYS == PHYS_MHD && a.add_force && rb.force) {
-      const int K = rb.force_K;
-      const int sl = signed_index(ls[j], G.ny), sm = signed_index(ms[j], G.nz);
-      if (kx <= K && sl >= -K && sl <= K && sm >= -K && sm <= K) {
-        const int64_t per = (int64_t)(K + 1) * (2 * K + 1) * (2 * K + 1);
-        const int64_t fi = ((int64_t)(sm + K) * (2 * K + 1) + (sl + K)) * (K + 1) + kx;
+    // addforcing! (src/pgen.jl:231-234) follows the MHD advection; HD discards it (:176-178)
+    if (PHYS == PHYS_MHD && a.add_force) {
+      if (rb.force) {  // seeded large-scale shell forcing
+        const int K = rb.force_K;
+        const int sl = signed_index(ls[j], G.ny), sm = signed_index(ms[j], G.nz);
+        if (kx <= K && sl >= -K && sl <= K && sm >= -K && sm <= K) {
+          const int64_t per = (int64_t)(K + 1) * (2 * K + 1) * (2 * K + 1);
+          const int64_t fi = ((int64_t)(sm + K) * (2 * K + 1) + (sl + K)) * (K + 1) + kx;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) R[c] = R[c] + rb.force[c * per + fi];
+          for (int c = 0; c < 3; ++c) R[c] = R[c] + rb.force[c * per + fi];
+        }
+      }
+      if (MODS && rb.fdense) {  // N97ForceDriving! (src/pgen/TaylorGreenDynamo.jl:12-16): N_u += fh
+        const gptr<const cx<T>> fdn = gmem<const cx<T>>(rb.fdense);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[c] = R[c] + ldst<true, T>(fdn + (int64_t)c * G.nc + q);
+      }
+      if (MODS && rb.ndf) {  // NDForceDriving! (src/pgen/NegativeDamping.jl:40): N_u += A R2C(f u), A Float64
+        const double A = rb.clk->nd_amp;
+        const gptr<const cx<T>> Pn = gmem<const cx<T>>(rb.buf[a.prod_sel]);
+        const int64_t pfs = field_stride(G, a.prod_sel);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const cx<T> F = ldst<true, T>(Pn + (int64_t)(9 + c) * pfs + qp);
+          R[c] = cx<T>{T((double)R[c].re + A * (double)F.re), T((double)R[c].im + A * (double)F.im)};
+        }
+      }
+      if (MODS && rb.a99) {
+        const uint64_t ctr = a.mode == SP_RHS ? (uint64_t)a.draw_ctr : (uint64_t)(8 * rb.clk->step + a.stage);
+        a99_add<T>(rb, kx, k, ctr, (uint64_t)q, R);
       }
     }
 
@@ -802,6 +964,26 @@ reduce_max_kernel(const RunBuf<T>* __restrict__ runs, int64_t nblocks, int advan
       c->step += 1;
     }
   }
+}
+
+// negative damping: A = P / (sum |u_i^2 f_i| dV) of the current right-hand side from the row
+// kernel's per-block Float64 sums (fixed summation order), one workgroup per run
+template <class T>
+__global__ void __launch_bounds__(256)
+nd_amp_kernel(const RunBuf<T>* __restrict__ runs, int64_t nblocks, double dV, int gate) {
+  __shared__ double red[256];
+  const RunBuf<T>& rb = runs[blockIdx.x];
+  if (!rb.ndf || gated_off(gate, rb)) return;
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int64_t i = t; i < nblocks; i += 256) s += rb.ndpart[i];
+  red[t] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) red[t] += red[t + st];
+    __syncthreads();
+  }
+  if (t == 0) rb.clk->nd_amp = rb.nd_P / (red[0] * dV);
 }
 
 // per-block max of u^2 and b^2 straight over vars (after the initial condition is set)

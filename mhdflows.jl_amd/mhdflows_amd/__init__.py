@@ -4,6 +4,13 @@ libmhdflows_amd.so (include/mhdflows_amd.h)."""
 
 from ._lib import MHDError, lib  # noqa: F401
 from .problem import (  # noqa: F401
+    A99ForceDriving,
+    N97ForceDriving,
+    NDForceDriving,
+    SetUpFk,
+    SetUpForcingField,
+    SetUpN97,
+    SetUpND,
     BiUpdate,
     Diagnostics,
     HDcalcN_advection,
@@ -30,5 +37,6 @@ from .problem import (  # noqa: F401
 __all__ = [
     "Problem", "SetUpProblemIC", "SetUpProblemIC_modes", "stepforward", "TimeIntegrator", "getCFL", "calcN",
     "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
-    "Diagnostics", "set_forcing", "get_sol_modes", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
+    "Diagnostics", "set_forcing", "get_sol_modes", "N97ForceDriving", "NDForceDriving",
+    "A99ForceDriving", "SetUpN97", "SetUpND", "SetUpFk", "SetUpForcingField", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
 ]

@@ -78,6 +78,9 @@ SIGNATURES = {
     "mhd_diagnostics": (_I, [_P, C.POINTER(Diag)]),
     "mhd_refresh_vars": (_I, [_P]),
     "mhd_set_forcing": (_I, [_P, C.c_uint64, C.c_double, C.c_double]),
+    "mhd_set_forcing_field": (_I, [_P, C.POINTER(_P)]),
+    "mhd_set_forcing_nd": (_I, [_P, C.c_double, C.POINTER(_P)]),
+    "mhd_set_forcing_a99": (_I, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "mhd_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),
     "mhd_set_variants": (_I, [_P, _P, _P]),

@@ -94,6 +94,22 @@ class _Grid:
         self.dy = T(T(Ly) / T(ny))
         self.dz = T(T(Lz) / T(nz))
 
+    def _axis(self, n, L, d):
+        # x_i = -L/2 + i dx rounded to T (FourierFlows' default origin, mirrored at utils.jl:43,58-60)
+        return (-L / 2 + np.arange(n) * float(d)).astype(self.T)
+
+    @property
+    def x(self):
+        return self._axis(self.nx, self.Lx, self.dx)
+
+    @property
+    def y(self):
+        return self._axis(self.ny, self.Ly, self.dy)
+
+    @property
+    def z(self):
+        return self._axis(self.nz, self.Lz, self.dz)
+
 
 class _Params:
     def __init__(self, nu, eta, nnu, neta, calcF):
@@ -376,6 +392,71 @@ def ProbDiagnostic(prob):
         return float(f"{x:.3g}")
 
     return (r3(d["E_kin"]), r3(d["E_mag"])) if prob.flag.b else r3(d["E_kin"])
+
+
+# ---- calcF! modules of the reference (src/pgen/*.jl), evaluated inside the device step ----
+def _device_module(name, doc):
+    def f(*args):
+        raise MHDError(-2, f"{name} runs inside the device step; pass it as calcF and call its setup")
+    f.__name__ = name
+    f.__doc__ = doc
+    f._device_forcing = True
+    return f
+
+
+N97ForceDriving = _device_module("N97ForceDriving", "``N97ForceDriving!`` (src/pgen/TaylorGreenDynamo.jl:12-16).")
+NDForceDriving = _device_module("NDForceDriving", "``NDForceDriving!`` (src/pgen/NegativeDamping.jl:23-44).")
+A99ForceDriving = _device_module("A99ForceDriving", "``A99ForceDriving!`` (src/pgen/A99ForceDriving.jl:33-60).")
+
+
+def _real_ptrs(prob, fields):
+    g = prob.grid
+    keep, ptrs = [], (C.c_void_p * 3)()
+    for c, a in enumerate(fields):
+        if a is None:
+            ptrs[c] = None
+            continue
+        a = np.ascontiguousarray(np.broadcast_to(a, (g.nz, g.ny, g.nx)), dtype=prob.T)
+        keep.append(a)
+        ptrs[c] = a.ctypes.data
+    return keep, ptrs
+
+
+def SetUpN97(prob, F0=1, kf=2):
+    """``SetUpN97!(prob; F0, kf)`` (src/pgen/TaylorGreenDynamo.jl:18-33): the Taylor-Green
+    forcing ``fx = F0 sin(kf x) cos(kf y) cos(kf z)``, ``fy = -F0 cos(kf x) sin(kf y) cos(kf z)``
+    evaluated in T on the grid, transformed on the device and added to N_u every call."""
+    g, T = prob.grid, prob.T
+    x = g.x[None, None, :]
+    y = g.y[None, :, None]
+    z = g.z[:, None, None]
+    kf, F0 = T(kf), T(F0)
+    fx = F0 * np.sin(kf * x) * np.cos(kf * y) * np.cos(kf * z)
+    fy = F0 * -np.cos(kf * x) * np.sin(kf * y) * np.cos(kf * z)
+    keep, ptrs = _real_ptrs(prob, [fx, fy, None])
+    check(lib().mhd_set_forcing_field(prob._h, ptrs))
+
+
+def SetUpForcingField(prob, fx=None, fy=None, fz=None):
+    """A static forcing field added to the velocity right-hand side (any real fields in
+    ``[z][y][x]`` order; the device-step counterpart of a calcF! that adds a fixed field)."""
+    keep, ptrs = _real_ptrs(prob, [fx, fy, fz])
+    check(lib().mhd_set_forcing_field(prob._h, ptrs))
+
+
+def SetUpND(prob, P, fx, fy, fz):
+    """``SetUpND!(prob, P, fx, fy, fz)`` (src/pgen/NegativeDamping.jl:16-22)."""
+    keep, ptrs = _real_ptrs(prob, [fx, fy, fz])
+    check(lib().mhd_set_forcing_nd(prob._h, float(P), ptrs))
+
+
+def SetUpFk(prob, kf=2, P=1, sigma2=1, seed=0, b=1.0, A=1.0, **kw):
+    """``SetUpFk(prob; kf, P, σ²)`` (src/pgen/A99ForceDriving.jl:93-127) with ``usr_vars.b`` and
+    ``usr_vars.A``; ``seed`` keys the counter-based draws that replace the reference's rand()."""
+    sigma2 = kw.pop("σ²", sigma2)
+    if kw:
+        raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
+    check(lib().mhd_set_forcing_a99(prob._h, int(seed), float(kf), float(P), float(sigma2), float(b), float(A)))
 
 
 def set_forcing(prob, seed, kf=2.0, amp=1.0):

@@ -42,6 +42,12 @@ How each part is pinned (checked by ``tests/test_oracle_*.py``)
   t = 0 right-hand side matches its closed form (pressure (cos 2x + cos 2y)(2 + cos 2z)/16).
 * Helicities: Arnold-Beltrami-Childress fields (curl f = k f) give H_kin = k E_kin and
   H_mag = E_mag / k, which fixes the signs of Curl and VectorPotential and the dV factor.
+* Forcing modules (``forcing``): N97 against its closed form and support; negative damping
+  against its injected power (sum u.F dV = P for f >= 0); A99 against its construction
+  (solenoidal, orthonormal e1/e2, zero kx = 0 plane, amplitude ~ sqrt(P)) and the uniformity
+  and independence of its counter-based draws.  A99's random stream itself is parity unpinned:
+  the reference draws unseeded rand(), so the build defines the draws (oracle and device share
+  them) and only the distribution and construction can be compared.
 * LSRK54: order conditions of the stability polynomial (Taylor series through z^4).
 * Alias index table: the kept ranges listed for the FourierFlows 1/3 rule.
 

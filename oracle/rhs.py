@@ -222,8 +222,13 @@ def calcN(N, sol, t, prob):
 
 
 def add_forcing(N, sol, t, prob):
-    """``addforcing!`` (``pgen.jl:231-234``) with the build's seeded large-scale forcing."""
+    """``addforcing!`` (``pgen.jl:231-234``): the problem's calcF!, i.e. one of the reference's
+    modules (``forcing.N97``, ``forcing.NegativeDamping``, ``forcing.A99``, each with ``apply``)
+    or the build's seeded large-scale forcing (``forcing.Forcing``)."""
     if prob.forcing is None:
+        return
+    if hasattr(prob.forcing, "apply"):
+        prob.forcing.apply(N, sol, t, prob)
         return
     Fh = prob.forcing.spectrum(prob.grid)  # (3, nz, ny, nkr), already scaled by N
     N[0:3] += Fh.astype(prob.grid.C)

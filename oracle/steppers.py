@@ -44,15 +44,20 @@ def rk4_step(prob):
     R2 = np.zeros_like(sol)
     R3 = np.zeros_like(sol)
     R4 = np.zeros_like(sol)
+    base = 8 * prob.step                          # draw counter of stochastic forcing: 8 step + stage
+    prob.force_counter = base
     calcN(R1, sol, t, prob)                       # substep 1 (dealiases prob.sol in place)
     _addlinear(R1, sol)
     sol1 = sol + half * R1                        # substepsol!(sol1, sol, RHS1, dt/2)
+    prob.force_counter = base + 1
     calcN(R2, sol1, T(t + half), prob)
     _addlinear(R2, sol1)
     sol1 = sol + half * R2
+    prob.force_counter = base + 2
     calcN(R3, sol1, T(t + half), prob)
     _addlinear(R3, sol1)
     sol1 = sol + dt * R3
+    prob.force_counter = base + 3
     calcN(R4, sol1, T(t + dt), prob)
     _addlinear(R4, sol1)
     sol += dt * (R1 / 6 + R2 / 3 + R3 / 3 + R4 / 6)   # RK4update!
@@ -71,6 +76,7 @@ def lsrk54_step(prob):
     S = np.zeros_like(sol)
     RHS = np.zeros_like(sol)
     for i in range(5):
+        prob.force_counter = 8 * prob.step + i
         calcN(RHS, sol, T(t + C[i] * dt), prob)
         _addlinear(RHS, sol)
         S[...] = A[i] * S + dt * RHS
