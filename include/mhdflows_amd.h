@@ -207,6 +207,20 @@ int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3);
  * mhd_calcN call of the handle; mode index = (m·ny + l)·nkr + kx. */
 int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, double sigma2, double b, double A);
 
+/* ---- checkpoint / restart ------------------------------------------------------ */
+/* savefile(prob, file_number; file_path_and_name) src/integrator.jl:259-288: write the real
+ * fields prob.vars as datasets i/j/k_velocity and i/j/k_mag_field (MHD) and clock.t as time.
+ * libhdf5 is not available, so the file is a small self-describing container (layout in
+ * mhd_ckpt.hip): named datasets with Julia-order dimensions, readable without the library.
+ * exact = 1 adds the spectral state sol and the clock (step, dt and the maxima getCFL! uses
+ * next), so that mhd_restart(.., exact = 1) continues bit for bit like an uninterrupted run
+ * (the reference's restart, from real fields, is not bit exact). */
+int32_t mhd_savefile(mhd_handle* h, const char* path, int32_t exact);
+/* Restart!(prob, file_path_and_name) src/integrator.jl:208-257: exact = 0 reads the real fields
+ * (converted to T), sets vars, R2Cs them into sol (Move_Data_to_Prob!, src/utils/utils.jl:172-176)
+ * and sets clock.t; exact = 1 restores sol and the whole clock saved with exact = 1. */
+int32_t mhd_restart(mhd_handle* h, const char* path, int32_t exact);
+
 /* ---- measurement -------------------------------------------------------------- */
 /* Autotuner: time every compiled variant of the hot launches (column-pass strip
  * widths, row-transform plans) with reps launches each, keep the fastest for this

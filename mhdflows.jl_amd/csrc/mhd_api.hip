@@ -313,6 +313,15 @@ int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, d
   return h->e->set_forcing_a99(0, seed, kf, P, sigma2, b, A);
 }
 
+int32_t mhd_savefile(mhd_handle* h, const char* path, int32_t exact) {
+  MHD_TRY(check_handle(h));
+  return save_checkpoint(h->e, 0, path, exact);
+}
+int32_t mhd_restart(mhd_handle* h, const char* path, int32_t exact) {
+  MHD_TRY(check_handle(h));
+  return restart_checkpoint(h->e, 0, path, exact);
+}
+
 int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len) {
   MHD_TRY(check_handle(h));
   return h->e->tune(reps, report, report_len);

@@ -849,6 +849,14 @@ class Engine : public EngineBase {
     if (step) *step = c.step;
     return 0;
   }
+  int clock_io(int run, RunClock* c, int write) override {
+    MHD_TRY(check_run(run));
+    return write ? write_clock(run, *c) : read_clock(run, c);
+  }
+  int precision_bytes() const override { return (int)sizeof(T); }
+  const Geom& geom() const override { return g; }
+  int nfields() const override { return nf; }
+
   int read_clock(int run, RunClock* c) {
     MHD_HIP(hipMemcpyAsync(c, rbh[run].clk, sizeof(RunClock), hipMemcpyDeviceToHost, st));
     MHD_HIP(hipStreamSynchronize(st));

@@ -61,6 +61,11 @@ class EngineBase {
   virtual int get_vars(int run, int f, void* host) = 0;
   virtual int device_ptrs(int run, void** sol, void** vars) = 0;
   virtual int get_clock(int run, double* t, double* dt, int64_t* step) = 0;
+  // the whole device clock of a run (write = 1 replaces it), for bit-exact checkpoints
+  virtual int clock_io(int run, RunClock* c, int write) = 0;
+  virtual int precision_bytes() const = 0;  // sizeof(T)
+  virtual const Geom& geom() const = 0;
+  virtual int nfields() const = 0;
   virtual int set_clock(int run, double t, double dt, int64_t step) = 0;
   virtual int set_cfl(int run, int use, double coef) = 0;
   virtual int calcN(void* N, void* sol, int advection_only) = 0;
@@ -90,6 +95,10 @@ class EngineBase {
 
 // Device bytes of one run.
 int64_t run_bytes(const Geom& g, int phys, int prec);
+
+// savefile / Restart! (src/integrator.jl:208-288): see mhd_ckpt.hip for the file layout
+int save_checkpoint(EngineBase* e, int run, const char* path, int exact);
+int restart_checkpoint(EngineBase* e, int run, const char* path, int exact);
 
 EngineBase* make_engine(int prec, const Geom& g, const double* L, int phys, int stepper,
                         const std::vector<RunParams>& runs, int device, hipStream_t stream, int* err);

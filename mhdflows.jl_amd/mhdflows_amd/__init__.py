@@ -4,6 +4,10 @@ libmhdflows_amd.so (include/mhdflows_amd.h)."""
 
 from ._lib import MHDError, lib  # noqa: F401
 from .problem import (  # noqa: F401
+    Restart,
+    read_checkpoint,
+    readMHDFlows,
+    savefile,
     A99ForceDriving,
     N97ForceDriving,
     NDForceDriving,
@@ -38,5 +42,6 @@ __all__ = [
     "Problem", "SetUpProblemIC", "SetUpProblemIC_modes", "stepforward", "TimeIntegrator", "getCFL", "calcN",
     "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
     "Diagnostics", "set_forcing", "get_sol_modes", "N97ForceDriving", "NDForceDriving",
-    "A99ForceDriving", "SetUpN97", "SetUpND", "SetUpFk", "SetUpForcingField", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
+    "A99ForceDriving", "SetUpN97", "SetUpND", "SetUpFk", "SetUpForcingField", "savefile", "Restart",
+    "readMHDFlows", "read_checkpoint", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
 ]

@@ -307,21 +307,25 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
     callables in ``diags`` and ``prob.usr_func`` hooks are called once per step and force
     a per-step loop on the host; without them the loop has no host synchronisation.
     """
+    file_path_and_name = ""
     if save:
         if len(save_loc) == 0 or len(filename) == 0 or dump_dt == 0:
             raise MHDError(-1, "Save Function Turned ON but save_loc/filename/dump_dt is not declared!\n")
-        raise MHDError(-2, "HDF5 output is not available in this build")
+        file_path_and_name = save_loc + filename                   # integrator.jl:44-51
+        savefile(prob, file_number, file_path_and_name=file_path_and_name)
+        file_number += 1
     if CFL_function is not nothingfunction:
         if usr_dt > 0.0:
             raise MHDError(-1, "User define both CFL_function and usr_dt")
         raise MHDError(-2, "user CFL functions run on the host and are not supported on the device loop")
     hooks = [f for f in prob.usr_func if f is not nothingfunction]
-    if not diags and not hooks:
+    if not diags and not hooks and not save:
         steps = C.c_int64()
         check(lib().mhd_integrate(prob._h, float(t0), int(N0), float(usr_dt), float(CFL_Coef), C.byref(steps)))
         return steps.value
-    # per-step loop with host hooks
+    # per-step loop with host hooks (integrator.jl:104-146)
     taken = 0
+    t_next_save = prob.clock.t + dump_dt                          # :75
     prob.clock.step = 0
     if usr_dt != 0.0:
         prob.clock.dt = usr_dt
@@ -330,10 +334,15 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
             getCFL(prob, Coef=CFL_Coef)
         stepforward(prob, 1)
         taken += 1
-        for d in diags:
+        for d in diags:                                           # increment!(diags)
             d(prob)
-        for f in hooks:
+        for f in hooks:                                           # usr_func
             f(prob)
+        if save and prob.clock.t >= t_next_save:                  # :136-141
+            ProbDiagnostic(prob)
+            savefile(prob, file_number, file_path_and_name=file_path_and_name)
+            t_next_save += dump_dt
+            file_number += 1
     return taken
 
 
@@ -457,6 +466,56 @@ def SetUpFk(prob, kf=2, P=1, sigma2=1, seed=0, b=1.0, A=1.0, **kw):
     if kw:
         raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
     check(lib().mhd_set_forcing_a99(prob._h, int(seed), float(kf), float(P), float(sigma2), float(b), float(A)))
+
+
+# ---- checkpoint / restart (src/integrator.jl:208-288, src/utils/IC.jl:245-257) ----
+CHECKPOINT_SUFFIX = ".mhdf"
+_DTYPES = {1: np.float32, 2: np.float64, 3: np.complex64, 4: np.complex128, 5: np.int64}
+
+
+def savefile(prob, file_number, file_path_and_name="", exact=False):
+    """``savefile(prob, file_number; file_path_and_name)`` (``src/integrator.jl:259-288``): writes
+    ``<file_path_and_name>_t_NNNN.mhdf`` with the datasets i/j/k_velocity, i/j/k_mag_field and
+    time (the library's container; libhdf5 is not available).  ``exact=True`` adds the spectral
+    state and clock for a bit-exact ``Restart``.  Returns the file name."""
+    name = f"{file_path_and_name}_t_{int(file_number):04d}{CHECKPOINT_SUFFIX}"
+    check(lib().mhd_savefile(prob._h, name.encode(), 1 if exact else 0))
+    return name
+
+
+def Restart(prob, file_path_and_name, exact=False):
+    """``Restart!(prob, file_path_and_name)`` (``src/integrator.jl:208-257``): real fields into
+    vars and sol, clock.t from the file; ``exact=True`` restores the saved spectral state and
+    clock instead (bit-exact continuation)."""
+    check(lib().mhd_restart(prob._h, str(file_path_and_name).encode(), 1 if exact else 0))
+
+
+def read_checkpoint(path):
+    """All datasets of a checkpoint file as numpy arrays (real fields in ``[z][y][x]`` order)."""
+    import struct
+    with open(path, "rb") as f:
+        buf = f.read()
+    if buf[:8] != b"MHDFLOWS":
+        raise MHDError(-1, f"{path} is not an MHDFlows checkpoint file")
+    ver, cnt = struct.unpack_from("<II", buf, 8)
+    if ver != 1:
+        raise MHDError(-1, f"{path}: unknown version {ver}")
+    out = {}
+    for i in range(cnt):
+        name, dt, nd, d0, d1, d2, d3, off, nb = struct.unpack_from("<48sII4QQQ", buf, 16 + 104 * i)
+        dims = (d0, d1, d2, d3)[:nd]
+        a = np.frombuffer(buf, dtype=_DTYPES[dt], count=nb // np.dtype(_DTYPES[dt]).itemsize, offset=off)
+        out[name.rstrip(b"\0").decode()] = a.reshape(tuple(reversed(dims))) if nd else a[0]
+    return out
+
+
+def readMHDFlows(FileName):
+    """``readMHDFlows(FileName)`` (``src/utils/IC.jl:245-257``): velocity and magnetic fields as
+    Float32 arrays and the time."""
+    d = read_checkpoint(FileName)
+    f32 = [np.asarray(d[k], dtype=np.float32) for k in ("i_velocity", "j_velocity", "k_velocity",
+                                                          "i_mag_field", "j_mag_field", "k_mag_field")]
+    return (*f32, d["time"])
 
 
 def set_forcing(prob, seed, kf=2.0, amp=1.0):
