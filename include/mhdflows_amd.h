@@ -207,6 +207,18 @@ int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3);
  * mhd_calcN call of the handle; mode index = (m·ny + l)·nkr + kx. */
 int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, double sigma2, double b, double A);
 
+/* ---- analysis of the current vars on the device ------------------------------- */
+/* spectralline(A) src/utils/MHDAnalysis.jl:236-255 for every vars field: *nbins = krmax =
+ * round(max|k| + 1); out (when not NULL, nbins_cap >= *nbins) receives nf + 1 rows of nbins:
+ * row f = sum of |R2C(vars f)|^2 over the half-spectrum modes with round(|k|) = r (round half
+ * to even, |k| in T), row nf = the number of those modes (the reference's kr marks the shells
+ * that have any).  Accumulated in Float64 (the reference sums in T). */
+int32_t mhd_spectra(mhd_handle* h, double* out, int32_t nbins_cap, int32_t* nbins);
+/* ScaleDecomposition(B1, B2, B3; kf = [k1, k2]) src/utils/MHDAnalysis.jl:55-82 of vars fields
+ * field0 .. field0 + ncomp - 1 (1 to 3 components of u or of b): modes with k1 <= |k| <= k2
+ * kept, back to real space; ncomp real fields of nx*ny*nz T to host_out. */
+int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, double k1, double k2, void* host_out);
+
 /* ---- checkpoint / restart ------------------------------------------------------ */
 /* savefile(prob, file_number; file_path_and_name) src/integrator.jl:259-288: write the real
  * fields prob.vars as datasets i/j/k_velocity and i/j/k_mag_field (MHD) and clock.t as time.

@@ -313,6 +313,20 @@ int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, d
   return h->e->set_forcing_a99(0, seed, kf, P, sigma2, b, A);
 }
 
+int32_t mhd_spectra(mhd_handle* h, double* out, int32_t nbins_cap, int32_t* nbins) {
+  MHD_TRY(check_handle(h));
+  const int need = h->e->spectrum_bins();
+  if (nbins) *nbins = need;
+  if (!out) return 0;
+  if (nbins_cap < need) return fail(E_INVALID, "out holds fewer than nbins bins per row");
+  return h->e->spectra(out, need);
+}
+int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, double k1, double k2, void* host_out) {
+  MHD_TRY(check_handle(h));
+  if (!host_out) return fail(E_INVALID, "host_out is NULL");
+  return h->e->scale_decomposition(field0, ncomp, k1, k2, host_out);
+}
+
 int32_t mhd_savefile(mhd_handle* h, const char* path, int32_t exact) {
   MHD_TRY(check_handle(h));
   return save_checkpoint(h->e, 0, path, exact);

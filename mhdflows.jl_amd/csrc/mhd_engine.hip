@@ -125,6 +125,7 @@ class Engine : public EngineBase {
         if (p) hipFree(p);
     if (dpart) hipFree(dpart);
     if (stage_) hipFree(stage_);
+    if (spec_) hipFree(spec_);
     if (rbd) hipFree(rbd);
     if (own_stream && st) hipStreamDestroy(st);
   }
@@ -770,6 +771,7 @@ class Engine : public EngineBase {
     if (bytes <= stage_cap_) return 0;
     MHD_HIP(hipStreamSynchronize(st));
     if (stage_) hipFree(stage_);
+    if (spec_) hipFree(spec_);
     stage_ = nullptr;
     stage_cap_ = 0;
     MHD_HIP(hipMalloc(&stage_, bytes));
@@ -1160,6 +1162,73 @@ class Engine : public EngineBase {
     MHD_TRY(xrow("xrow_r2c", ra, ROW_R2C, 3, 0));
     MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, 3, false));
     MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    return 0;
+  }
+
+  // ---- analysis (src/utils/MHDAnalysis.jl) ----------------------------------------------
+  // krmax = round(Int, maximum(sqrt(Krsq)) + 1) (MHDAnalysis.jl:244), |k| in T
+  int spectrum_bins() const override {
+    const T qx = (T)((double)(g.nx / 2) * g.multx), qy = (T)((double)(g.ny / 2) * g.multy);
+    const T qz = (T)((double)(g.nz / 2) * g.multz);
+    const T kmax = std::sqrt((qx * qx + qy * qy) + qz * qz);
+    return (int)std::nearbyint((double)(T)(kmax + T(1)));
+  }
+
+  double* spec_ = nullptr;  // shell sums of one analysis call, grown on demand
+  size_t spec_cap_ = 0;
+
+  // spectralline of every vars field: out[f * nbins + r] = sum |R2C(vars f)|^2 over the modes with
+  // round(|k|) = r, out[nf * nbins + r] = their number
+  int spectra(double* host_out, int nbins) override {
+    if (nr_ != 1) return fail(E_INVALID, "spectra act on single-run problems");
+    const int need = spectrum_bins();
+    if (nbins < need) return fail(E_INVALID, "spectra need " + std::to_string(need) + " bins");
+    const size_t per = (size_t)4 * nbins;  // 3 fields + counts per transformed triple
+    const size_t total = per * (size_t)((nf + 2) / 3);
+    if (total > spec_cap_) {
+      MHD_HIP(hipStreamSynchronize(st));
+      if (spec_) (void)hipFree(spec_);
+      spec_ = nullptr;
+      spec_cap_ = 0;
+      MHD_HIP(hipMalloc(&spec_, sizeof(double) * total));
+      spec_cap_ = total;
+    }
+    MHD_HIP(hipMemsetAsync(spec_, 0, sizeof(double) * total, st));
+    for (int f0 = 0; f0 < nf; f0 += 3) {
+      MHD_TRY(r2c_fields(f0));
+      MHD_TRY(timed("shell", 0, [&] { return launch_shell<T>(g, rbh[0].buf[B_WORK], 3, spec_ + per * (f0 / 3), nbins, st); }));
+    }
+    std::vector<double> h(total);
+    MHD_HIP(hipMemcpyAsync(h.data(), spec_, sizeof(double) * total, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    for (int f = 0; f < nf; ++f)
+      std::memcpy(host_out + (size_t)f * nbins, h.data() + per * (f / 3) + (size_t)(f % 3) * nbins, sizeof(double) * nbins);
+    std::memcpy(host_out + (size_t)nf * nbins, h.data() + 3 * (size_t)nbins, sizeof(double) * nbins);
+    return 0;
+  }
+
+  // ScaleDecomposition of vars fields f0 .. f0 + ncomp - 1 (within one transformed triple):
+  // keep k1 <= |k| <= k2, back to real space, ncomp real fields to host_out
+  int scale_decomposition(int f0, int ncomp, double k1, double k2, void* host_out) override {
+    if (nr_ != 1) return fail(E_INVALID, "ScaleDecomposition acts on single-run problems");
+    if (ncomp < 1 || ncomp > 3 || f0 < 0 || f0 + ncomp > nf || (f0 / 3) != ((f0 + ncomp - 1) / 3))
+      return fail(E_INVALID, "fields must be 1 to 3 components of u or of b");
+    const int base = f0 - f0 % 3;
+    MHD_TRY(r2c_fields(base));
+    MHD_TRY(timed("band", 0, [&] { return launch_band<T>(g, rbh[0].buf[B_WORK], 3, std::min(k1, k2), std::max(k1, k2), st); }));
+    MHD_TRY(colpass("zpass_inv", 2, 1, B_WORK, B_WORK, 0, 0, 3, false));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, 3, false));
+    RowArgs ra = row_args();
+    ra.in_sel = B_WORK;
+    ra.in_f0 = 0;
+    ra.in_ncols = g.nkr;
+    ra.real_sel = 1;
+    ra.real_out_f0 = 0;
+    ra.write_vars = 1;
+    MHD_TRY(xrow("xrow_c2r", ra, ROW_C2R, 3, 0));
+    MHD_HIP(hipMemcpyAsync(host_out, rbh[0].rscr + (int64_t)(f0 - base) * g.nr, sizeof(T) * g.nr * ncomp,
+                           hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
     return 0;
   }
 

@@ -77,6 +77,10 @@ class EngineBase {
   virtual int set_graph(int enable) = 0;
   virtual int diagnostics(double* out8) = 0;
   virtual int refresh_vars() = 0;
+  // analysis of vars (src/utils/MHDAnalysis.jl): shell spectra and band filter
+  virtual int spectrum_bins() const = 0;
+  virtual int spectra(double* host_out, int nbins) = 0;
+  virtual int scale_decomposition(int f0, int ncomp, double k1, double k2, void* host_out) = 0;
   virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;
   virtual int set_forcing_field(int run, const void* const* f3) = 0;
   virtual int set_forcing_nd(int run, double P, const void* const* f3) = 0;

@@ -1137,6 +1137,64 @@ diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ 
   dst[idx] = c0; dst[G.wnc + idx] = c1; dst[2 * G.wnc + idx] = c2;
 }
 
+// ---------------------------------------------------------------------------
+// analysis (src/utils/MHDAnalysis.jl): shell spectra and band filters on the work buffer
+// ---------------------------------------------------------------------------
+// |k| of a mode in T, as the reference's grid arrays give it: sqrt((kx^2 + ky^2) + kz^2)
+template <class T>
+__device__ __forceinline__ T mode_kabs(const Geom& G, int kx, int l, int m) {
+  const T qx = T((double)kx * G.multx), qy = T((double)signed_index(l, G.ny) * G.multy);
+  const T qz = T((double)signed_index(m, G.nz) * G.multz);
+  return sqrt((qx * qx + qy * qy) + qz * qz);
+}
+
+// spectralline (MHDAnalysis.jl:236-255): shell r = round(|k|) (half to even) of every half-
+// spectrum mode of ncomp work fields; out[c * nbins + r] += |A_k|^2 and out[ncomp * nbins + r]
+// counts the modes.  Accumulated in Float64 through LDS and global atomics (the summation
+// order, hence the last bits, is not fixed).  LDS: (ncomp + 1) * nbins doubles.
+template <class T>
+__global__ void __launch_bounds__(256)
+shell_kernel(Geom G, const cx<T>* __restrict__ work, int ncomp, double* __restrict__ out, int nbins) {
+  extern __shared__ double hist[];
+  const int t = threadIdx.x;
+  const int nh = (ncomp + 1) * nbins;
+  for (int i = t; i < nh; i += 256) hist[i] = 0.0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < G.nc; i += (int64_t)gridDim.x * 256) {
+    const int kx = (int)(i % G.nkr);
+    const int64_t rest = i / G.nkr;
+    const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+    const T k = mode_kabs<T>(G, kx, l, m);
+    const int r = (int)rint((double)k);
+    if (r < 0 || r >= nbins) continue;
+    const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.wrs + kx;
+    for (int c = 0; c < ncomp; ++c) {
+      const cx<T> v = work[(int64_t)c * G.wnc + idx];
+      atomicAdd(&hist[c * nbins + r], (double)v.re * (double)v.re + (double)v.im * (double)v.im);
+    }
+    atomicAdd(&hist[ncomp * nbins + r], 1.0);
+  }
+  __syncthreads();
+  for (int i = t; i < nh; i += 256)
+    if (hist[i] != 0.0) atomicAdd(&out[i], hist[i]);
+}
+
+// ScaleDecomposition (MHDAnalysis.jl:55-82): keep the modes with k1 <= |k| <= k2 of ncomp work
+// fields (|k| in T), zero the rest, in place
+template <class T>
+__global__ void __launch_bounds__(256)
+band_kernel(Geom G, cx<T>* __restrict__ work, int ncomp, double k1, double k2) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= G.nc) return;
+  const int kx = (int)(i % G.nkr);
+  const int64_t rest = i / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  const T k = mode_kabs<T>(G, kx, l, m);
+  if ((double)k >= k1 && (double)k <= k2) return;
+  const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.wrs + kx;
+  for (int c = 0; c < ncomp; ++c) work[(int64_t)c * G.wnc + idx] = cx<T>{T(0), T(0)};
+}
+
 // scatter sparse half-spectrum entries (kx, l, m) of nf fields into x (already zeroed)
 template <class T>
 __global__ void __launch_bounds__(256)

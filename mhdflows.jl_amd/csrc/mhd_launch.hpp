@@ -90,6 +90,10 @@ hipError_t launch_zero(cx<T>* x, int64_t n, hipStream_t s);
 template <class T>
 hipError_t launch_unpad(const Geom& g, const cx<T>* work_field, cx<T>* dst, hipStream_t s);
 template <class T>
+hipError_t launch_shell(const Geom& g, const cx<T>* work, int ncomp, double* out, int nbins, hipStream_t s);
+template <class T>
+hipError_t launch_band(const Geom& g, cx<T>* work, int ncomp, double k1, double k2, hipStream_t s);
+template <class T>
 hipError_t launch_nd_amp(const RunBuf<T>* runs, int nruns, int64_t nblocks, double dV, int gate, hipStream_t s);
 
 enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3, SK_PRUNE = 4 };

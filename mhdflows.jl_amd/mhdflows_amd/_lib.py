@@ -82,6 +82,8 @@ SIGNATURES = {
     "mhd_set_forcing_nd": (_I, [_P, C.c_double, C.POINTER(_P)]),
     "mhd_set_forcing_a99": (_I, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "mhd_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
+    "mhd_spectra": (_I, [_P, _P, _I, C.POINTER(_I)]),
+    "mhd_scale_decomposition": (_I, [_P, _I, _I, C.c_double, C.c_double, _P]),
     "mhd_savefile": (_I, [_P, C.c_char_p, _I]),
     "mhd_restart": (_I, [_P, C.c_char_p, _I]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),

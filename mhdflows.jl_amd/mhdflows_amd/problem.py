@@ -468,6 +468,55 @@ def SetUpFk(prob, kf=2, P=1, sigma2=1, seed=0, b=1.0, A=1.0, **kw):
     check(lib().mhd_set_forcing_a99(prob._h, int(seed), float(kf), float(P), float(sigma2), float(b), float(A)))
 
 
+# ---- analysis of the current vars on the device (src/utils/MHDAnalysis.jl) ----
+_FIELDS = ("ux", "uy", "uz", "bx", "by", "bz")
+
+
+def _field_index(prob, field):
+    f = _FIELDS.index(field) if isinstance(field, str) else int(field)
+    if not 0 <= f < prob.nf:
+        raise MHDError(-1, f"field {field!r} is not a field of this problem")
+    return f
+
+
+def spectra(prob):
+    """Shell spectra of every vars field: ``{name: Pk}`` plus ``"count"`` (modes per shell),
+    each of length krmax (see ``spectralline``)."""
+    nb = C.c_int32()
+    check(lib().mhd_spectra(prob._h, None, 0, C.byref(nb)))
+    out = np.zeros((prob.nf + 1, nb.value), np.float64)
+    check(lib().mhd_spectra(prob._h, out.ctypes.data_as(C.c_void_p), nb.value, C.byref(nb)))
+    d = {_FIELDS[f]: out[f] for f in range(prob.nf)}
+    d["count"] = out[prob.nf]
+    return d
+
+
+def spectralline(prob, field="ux"):
+    """``spectralline(A)`` (``src/utils/MHDAnalysis.jl:236-255``) of a vars field: ``Pk[r]`` is the
+    sum of ``|A_k|^2`` over the half-spectrum modes with ``round(|k|) = r`` (0-based r), and
+    ``kr[r] = r + 1`` (the reference's 1-based shell label) for every shell that holds a mode,
+    0 otherwise.  Pk is accumulated in Float64."""
+    d = spectra(prob)
+    Pk = d[_FIELDS[_field_index(prob, field)]]
+    kr = np.where(d["count"] > 0, np.arange(1, Pk.size + 1), 0).astype(prob.T)
+    return Pk, kr
+
+
+def ScaleDecomposition(prob, fields=("ux", "uy", "uz"), kf=(1, 5)):
+    """``ScaleDecomposition(B1, B2, B3; kf)`` (``src/utils/MHDAnalysis.jl:17-82``) of vars fields:
+    the modes with ``kf[1] <= |k| <= kf[2]`` back in real space, one ``[z][y][x]`` array each."""
+    if isinstance(fields, (str, int)):
+        fields = (fields,)
+    idx = [_field_index(prob, f) for f in fields]
+    if idx != list(range(idx[0], idx[0] + len(idx))):
+        raise MHDError(-1, "fields must be consecutive components of u or of b")
+    g = prob.grid
+    out = np.empty((len(idx), g.nz, g.ny, g.nx), prob.T)
+    check(lib().mhd_scale_decomposition(prob._h, idx[0], len(idx), float(min(kf)), float(max(kf)),
+                                        out.ctypes.data_as(C.c_void_p)))
+    return tuple(out)
+
+
 # ---- checkpoint / restart (src/integrator.jl:208-288, src/utils/IC.jl:245-257) ----
 CHECKPOINT_SUFFIX = ".mhdf"
 _DTYPES = {1: np.float32, 2: np.float64, 3: np.complex64, 4: np.complex128, 5: np.int64}
