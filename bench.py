@@ -5,12 +5,16 @@ Float32, RK4, nu = eta = 1e-3, dt from getCFL! with CFL 0.3 before every step,
 seeded solenoidal initial fields with spectrum k^4 exp(-2 (k/4)^2), |k| <= 16,
 u_rms = b_rms = 1, cross-helicity 0.  Inputs are resident in HBM before timing.
 
-One JSON line on stdout.  `roofline` is measured with HIP events around every
-launch of a separate profiled pass; `cpu_baseline` times the oracle's literal
-(reference-structured, 36 FFTs per right-hand side) path on the host cores.
-`value` uses the library's default dealiased steps (every step of the integration but
-the last updates the kept modes only; the final state is the same bit for bit);
-`every_step_full` repeats the timing with every step computed in full.
+One JSON line on stdout.  `value` is the rate of one call of --steps steps with the
+library's default dealiased steps (every step of a call but the last updates the kept modes
+only; the final state is the same bit for bit).  `roofline` is the dominant launch:
+algorithmic bytes per launch over its average duration, both measured with HIP events
+around every launch of a separate profiled call, and its HBM traffic from the committed
+rocprofv3 counter summary.  `step_roofline` gives the whole step's bytes per step time:
+the executed launches' algorithmic bytes, the counters' HBM bytes (frac = HBM rate / 8 TB/s),
+for the timed call and for a call of full steps; the fixed byte model of the reference
+algorithm is reported only as an equivalent rate.  `cpu_baseline` times the CPU
+restatement of the reference path (36 FFTs per right-hand side) on the host cores.
 """
 
 import argparse
@@ -26,40 +30,85 @@ for _p in (ROOT, os.path.join(ROOT, "mhdflows.jl_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+VARIANT_NAMES = ("zpass_inv", "ypass_inv", "ypass_fwd", "zpass_fwd", "xrow_fused")
 COUNTER_PROFILE = os.path.join("profiles", "mhd256_fp32_rk4_launches.md")
 
 
-def counter_step_traffic(variants, profile=None):
-    """HBM bytes of one right-hand side's six launches x 4 (an RK4 step) from the committed
-    counter summary (the source string says whether its launch variants match this run's)."""
-    profile = profile or COUNTER_PROFILE
+def counter_blob(profile):
+    """The machine-readable block of a committed rocprofv3 counter summary, or None."""
     try:
         txt = open(os.path.join(ROOT, profile)).read()
         blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
+        return None
+    if not all("dealiased" in r and "full" in r for r in blob.get("launches", {}).values()):
+        return None
+    blob["file"] = profile
+    return blob
+
+
+def counter_step_dram(blob, variants, steps_per_call):
+    """HBM bytes per RK4 step (4 right-hand sides of the six launches) of a call of
+    `steps_per_call` steps, whose last step is full and the others dealiased, from the counter
+    summary's per-launch FETCH_SIZE/WRITE_SIZE bytes of both kinds of step; and its source."""
+    if blob is None:
         return None, None
-    tot = 4 * sum(rec["traffic_bytes"] for rec in blob["launches"].values())
+    d = sum(rec["dealiased"]["traffic_bytes"] for rec in blob["launches"].values())
+    f = sum(rec["full"]["traffic_bytes"] for rec in blob["launches"].values())
+    s = max(int(steps_per_call), 1)
     same = "the same launch variants" if blob.get("variants") == variants else \
         f"launch variants {blob.get('variants')} (this run chose others)"
-    return round(tot), (f"{profile}: 4 x the six launches' FETCH_SIZE/WRITE_SIZE bytes "
-                        f"(dealiased and full steps in the bench's proportion), taken with {same}")
+    src = (f"{blob['file']}: 4 x the six launches' calibrated FETCH_SIZE + WRITE_SIZE bytes, "
+           f"{s - 1} dealiased + 1 full step per call, taken with {same}")
+    return {"dealiased": round(4 * d), "full": round(4 * f), "call": round(4 * ((s - 1) * d + f) / s)}, src
 
 
-def counter_traffic(launch, variants):
-    """HBM bytes per launch of `launch` from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration recorded there),
-    or None when the summary lacks the launch or was taken with another variant of it."""
-    try:
-        txt = open(os.path.join(ROOT, COUNTER_PROFILE)).read()
-        blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
-    except (OSError, IndexError, ValueError):
+def counter_launch_traffic(blob, launch, variants, steps_per_call):
+    """HBM bytes per launch of `launch` in a call of `steps_per_call` steps (the same mix of
+    dealiased and full steps), or None when the summary lacks it or used another variant."""
+    if blob is None:
         return None, None
     rec = blob.get("launches", {}).get(launch)
     if rec is None or blob.get("variants", {}).get(launch) != variants.get(launch):
         return None, None
-    src = (f"{COUNTER_PROFILE}: FETCH_SIZE x{blob['calibration']['fetch_factor']} + WRITE_SIZE "
-           f"x{blob['calibration']['write_factor']} per dispatch of {rec['kernel']} (copy-kernel calibrated)")
-    return rec["traffic_bytes"], src
+    s = max(int(steps_per_call), 1)
+    t = ((s - 1) * rec["dealiased"]["traffic_bytes"] + rec["full"]["traffic_bytes"]) / s
+    cal = blob["calibration"]
+    src = (f"{blob['file']}: FETCH_SIZE x{cal['fetch_factor']} + WRITE_SIZE x{cal['write_factor']} per "
+           f"dispatch of {rec['kernel']} (copy-kernel calibrated), {s - 1} dealiased + 1 full step")
+    return round(t), src
+
+
+def step_roofline(model_bytes, steps_per_call, ms_call, exec_call, ms_full, exec_full, dram, dram_src):
+    """Whole-step rates: the bytes the executed launches move (the library's per-launch byte
+    model of what each launch reads and writes) and the HBM bytes the counters saw, per step
+    time; `frac` is the HBM rate of the timed call against the 8 TB/s peak.  The fixed
+    three-pass model of the reference algorithm is kept as a labelled equivalent rate."""
+    def leg(ms, alg, hbm):
+        out = {"ms_per_step": round(ms, 4), "executed_alg_bytes_per_step": round(alg),
+               "executed_alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
+        if hbm is not None:
+            out["dram_bytes_per_step"] = hbm
+            out["dram_GBps"] = round(hbm / (ms * 1e-3) / 1e9, 1)
+            out["frac"] = round(hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        else:
+            out["frac"] = round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        return out
+    call = leg(ms_call, exec_call, dram["call"] if dram else None)
+    call["steps_per_call"] = int(steps_per_call)
+    out = {"peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": call["frac"],
+           "frac_basis": "HBM bytes per step (counters)" if dram else "executed algorithmic bytes per step",
+           "timed_call": call}
+    if ms_full is not None:
+        out["every_step_full"] = leg(ms_full, exec_full, dram["full"] if dram else None)
+    out["dram_source"] = dram_src
+    out["model_equivalent"] = {
+        "model_bytes_per_step": model_bytes,
+        "GBps_timed_call": round(model_bytes / (ms_call * 1e-3) / 1e9, 1),
+        "GBps_every_step_full": round(model_bytes / (ms_full * 1e-3) / 1e9, 1) if ms_full else None,
+        "note": "fixed three-pass byte model of the reference algorithm (MHD RK4: 438 V_c + 120 V_r) "
+                "over the step time: a rate comparable across implementations, not a fraction of peak"}
+    return out
 
 
 def log(*a):
@@ -100,27 +149,84 @@ def executed_bytes_per_step(prof, nsteps):
     return sum(k["bytes"] for k in prof) / nsteps
 
 
-def cpu_baseline(fields, n, steps=1):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _ranges(cpus):
+    cpus, out, i = sorted(cpus), [], 0
+    while i < len(cpus):
+        j = i
+        while j + 1 < len(cpus) and cpus[j + 1] == cpus[j] + 1:
+            j += 1
+        out.append(f"{cpus[i]}-{cpus[j]}" if j > i else str(cpus[i]))
+        i = j + 1
+    return ",".join(out)
+
+
+def cpu_baseline(sizes=(32, 64, 128, 256), timed=3):
+    """The CPU restatement of the reference path (the oracle's literal mode: the reference's
+    operation structure, 36 real 3D FFTs per right-hand side, pocketfft with one worker per
+    core, single-threaded elementwise work), timed on this host's cores.  32^3 runs the 3D
+    Taylor-Green problem (nu = eta = 1e-2, dt = 1e-3, 100 steps); the larger grids run the
+    bench workload (nu = eta = 1e-3, CFL 0.3) for one warm-up and `timed` timed steps.
+    `value` is the 256^3 rate, the GPU workload's."""
     import numpy as np
+    from mhdflows_amd import ic
     from oracle import OracleProblem
     from oracle import rhs as orhs
-    from oracle.integrator import time_integrator
-    cores = min(16, os.cpu_count() or 1)
+    from oracle.integrator import run_steps, time_integrator
+    try:
+        aff = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = list(range(os.cpu_count() or 1))
+    cores = max(1, min(16, len(aff)))   # the host share of one GPU is 16 cores
     orhs.WORKERS = cores
-    o = OracleProblem(nx=n, nu=1e-3, eta=1e-3, B_field=True, T=np.float32, mode="literal")
-    o.set_ic(*fields)
-    t0 = time.perf_counter()
-    time_integrator(o, math.inf, steps - 1, CFL_Coef=0.3)
-    dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} RK4 step(s) of the {n}^3 FP32 MHD workload through the oracle's literal "
-                      f"reference-structured path (36 FFTs per right-hand side, scipy.fft workers={cores}, "
-                      f"single-threaded elementwise), {dt:.1f} s"}
+    readme = {32: 20.4, 64: 5.56, 128: 0.671, 256: 0.0690}   # README.md:77, Ryzen 7 5800X, 8 threads
+    rows = {}
+    for n in sizes:
+        if n == 32:
+            u, b = ic.taylor_green(32, dtype=np.float64)
+            o = OracleProblem(nx=32, nu=1e-2, eta=1e-2, B_field=True, T=np.float32, dt=1e-3, mode="literal")
+            o.set_ic(*u, *b)
+            t0 = time.perf_counter()
+            run_steps(o, 100)
+            dt, nst = time.perf_counter() - t0, 100
+            what = "Taylor-Green, dt 1e-3, 100 steps"
+        else:
+            fields = ic.solenoidal_modes(2024, k0=4.0, kmax=16.0).real_fields(n, dtype=np.float64)
+            o = OracleProblem(nx=n, nu=1e-3, eta=1e-3, B_field=True, T=np.float32, mode="literal")
+            o.set_ic(*fields)
+            del fields
+            time_integrator(o, math.inf, 0, CFL_Coef=0.3)
+            t0 = time.perf_counter()
+            time_integrator(o, math.inf, timed - 1, CFL_Coef=0.3)
+            dt, nst = time.perf_counter() - t0, timed
+            what = f"bench workload, CFL 0.3, 1 warm-up + {timed} timed steps"
+        rows[str(n)] = {"steps_per_s": round(nst / dt, 4), "zone_updates_per_s": round(nst * n ** 3 / dt, 1),
+                        "seconds": round(dt, 2), "run": what, "readme_ryzen_5800x_steps_per_s": readme.get(n)}
+        log(f"bench: CPU {n}^3 {rows[str(n)]}")
+        del o
+    top = rows[str(max(sizes))]
+    return {"value": top["steps_per_s"], "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{max(sizes)}^3 FP32 MHD RK4, {top['run']}, through the CPU restatement (pocketfft) of the "
+                      f"reference path in its literal operation structure (36 FFTs per right-hand side), "
+                      f"scipy.fft workers={cores}, elementwise single-threaded",
+            "label": "CPU restatement (pocketfft), not the Julia/FFTW reference",
+            "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(), "affinity": _ranges(aff),
+            "sizes": rows}
 
 
-def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
+def large_bench(M, n=1024, steps=10, warmup=2, variants=None, full_steps=False, retime_full=True):
     """The 1024^3 Float32 MHD RK4 step (nu = eta = 2e-4, CFL 0.3, seeded |k| <= 16 fields
-    set as sparse modes on the device), steps/s and the step roofline."""
+    set as sparse modes on the device): steps/s of one call of `steps` steps and the step
+    roofline of that call and of a call of full steps."""
     import ctypes as C
     import numpy as np
     from mhdflows_amd import ic
@@ -135,6 +241,8 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
         M.variants(p, variants)
     else:
         M.autotune(p, 2)
+    if full_steps:
+        M.set_step_pruning(p, False)
     M.TimeIntegrator(p, math.inf, warmup - 1, CFL_Coef=0.3)
     hip().hipDeviceSynchronize()
     t0 = time.perf_counter()
@@ -142,31 +250,33 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None):
     hip().hipDeviceSynchronize()
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / steps
-    full_model = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
-    exec_bytes = executed_bytes_per_step(profile(M, p, steps), steps)
-    vn = M.variants(p)
-    dram = counter_step_traffic({"zpass_inv": vn[0], "ypass_inv": vn[1], "ypass_fwd": vn[2], "zpass_fwd": vn[3],
-                                 "xrow_fused": vn[4]}, os.path.join("profiles", "mhd1024_fp32_rk4_launches.md"))
-    gbs = full_model / (ms * 1e-3) / 1e9
+    model = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
+    prof = profile(M, p, steps)
+    exec_call = executed_bytes_per_step(prof, steps)
     d = M.Diagnostics(p)
-    M.set_step_pruning(p, False)
-    hip().hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
-    hip().hipDeviceSynchronize()
-    wall_full = time.perf_counter() - t0
-    M.set_step_pruning(p, True)
-    gbs_full = full_model / (wall_full / steps) / 1e9
-    out = {"grid": n, "steps": steps, "value": round(steps / wall, 4), "unit": "steps/s",
-           "ms_per_step": round(ms, 2), "variants": M.variants(p),
-           "step_roofline": {"model_bytes_per_step": full_model, "achieved": round(gbs, 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                             "executed_alg_bytes_per_step": round(exec_bytes),
-                             "executed_alg_GBps": round(exec_bytes / (ms * 1e-3) / 1e9, 1),
-                             "dram_bytes_per_step": dram[0], "dram_source": dram[1]},
-           "every_step_full": {"value": round(steps / wall_full, 4), "ms_per_step": round(wall_full * 1e3 / steps, 2),
-                               "model_bytes_per_step": full_model, "achieved": round(gbs_full, 1),
-                               "frac": round(gbs_full / HBM_PEAK_GBS, 4)},
+    ms_full = exec_full = None
+    if retime_full and not full_steps:
+        M.set_step_pruning(p, False)
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        M.TimeIntegrator(p, math.inf, steps - 1, CFL_Coef=0.3)
+        hip().hipDeviceSynchronize()
+        ms_full = (time.perf_counter() - t0) * 1e3 / steps
+        exec_full = executed_bytes_per_step(profile(M, p, steps), steps)
+        M.set_step_pruning(p, True)
+    vn = M.variants(p)
+    named = dict(zip(VARIANT_NAMES, vn))
+    blob = counter_blob(os.path.join("profiles", "mhd1024_fp32_rk4_launches.md"))
+    dram, src = counter_step_dram(blob, named, 1 if full_steps else steps)
+    if full_steps and dram:
+        dram["call"] = dram["full"]
+    out = {"grid": n, "workload": "incompressible MHD 1024^3 Float32 RK4, nu=eta=2e-4, CFL 0.3 per step, "
+                                  "seeded solenoidal k^4 exp(-2(k/4)^2) fields |k|<=16 set as device modes",
+           "steps": steps, "value": round(steps / wall, 4), "unit": "steps/s", "ms_per_step": round(ms, 2),
+           "every_step_full": full_steps, "variants": named,
+           "step_roofline": step_roofline(model, steps, ms, exec_call, ms_full, exec_full, dram, src),
+           "kernels": {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4)}
+                       for k in prof},
            "finite": not d["has_nan"]}
     del p
     return out
@@ -325,15 +435,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-full", action="store_true", help="skip the every-step-full timing")
+    ap.add_argument("--full-steps", action="store_true",
+                    help="compute every step in full for the whole run (counter runs of full steps)")
     ap.add_argument("--profile-steps", type=int, default=0,
-                    help="steps profiled per launch (default: --steps, the timed run's mix of steps)")
+                    help="steps of the profiled call (default: --steps, the timed call's mix of steps)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-sizes", type=str, default="32,64,128,256")
     ap.add_argument("--variants", type=str, default="",
                     help="fixed launch variants zinv,yinv,yfwd,zfwd,row instead of autotuning")
     ap.add_argument("--no-copy", action="store_true", help="skip the reference copy-bandwidth measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 1024^3 measurement")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run the 1024^3 leg alone (its own JSON line); --steps/--warmup apply to it")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -345,25 +459,36 @@ def main():
               "ensembles by placing runs on GPUs)", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes as C
     import numpy as np
     import mhdflows_amd as M
     from mhdflows_amd import ic
     from mhdflows_amd.devmem import hip
 
+    fixed = [int(v) for v in args.variants.split(",")] if args.variants else None
+    if args.large_only:
+        log("bench: 1024^3 Float32 MHD RK4 alone")
+        out = large_bench(M, steps=args.steps, warmup=args.warmup, variants=fixed, full_steps=args.full_steps,
+                          retime_full=not args.no_full)
+        print(json.dumps(out), flush=True)
+        return
+
     n = args.n
     log(f"bench: building {n}^3 FP32 MHD problem")
     p, fields = make_problem(M, ic, n, np.float32)
-    cfg = p.cfg
-    import ctypes as C
-    step_bytes = M.lib().mhd_step_bytes_model(C.byref(cfg))
+    del fields
+    step_bytes = M.lib().mhd_step_bytes_model(C.byref(p.cfg))
 
-    if args.variants:
-        chosen = M.variants(p, [int(v) for v in args.variants.split(",")])
+    if fixed:
+        chosen = M.variants(p, fixed)
     else:
         log("bench: autotuning launch variants")
         tune_report = M.autotune(p, 5)
         log(tune_report.strip())
         chosen = M.variants(p)
+    chosen_named = dict(zip(VARIANT_NAMES, chosen))
+    if args.full_steps:
+        M.set_step_pruning(p, False)
 
     log(f"bench: warmup {args.warmup} steps")
     M.TimeIntegrator(p, math.inf, max(args.warmup, 1) - 1, CFL_Coef=0.3)
@@ -383,32 +508,33 @@ def main():
     nprof = args.profile_steps or args.steps
     log(f"bench: profiling {nprof} steps with per-launch HIP events")
     prof = profile(M, p, nprof)
-    exec_bytes = executed_bytes_per_step(prof, nprof)
-    wall_full = None
-    if not args.no_full:
+    exec_call = executed_bytes_per_step(prof, nprof)
+    ms_full = exec_full = None
+    if not args.no_full and not args.full_steps:
         M.set_step_pruning(p, False)
         hip().hipDeviceSynchronize()
         t0 = time.perf_counter()
         M.TimeIntegrator(p, math.inf, args.steps - 1, CFL_Coef=0.3)
         hip().hipDeviceSynchronize()
-        wall_full = time.perf_counter() - t0
+        ms_full = (time.perf_counter() - t0) * 1e3 / args.steps
+        exec_full = executed_bytes_per_step(profile(M, p, nprof), nprof)
         M.set_step_pruning(p, True)
-        log(f"bench: every step full: {args.steps / wall_full:.2f} steps/s")
+        log(f"bench: every step full: {1e3 / ms_full:.2f} steps/s")
     kern = [k for k in prof if k["bytes"] > 0]
     dom = max(kern, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    chosen_named = {"zpass_inv": chosen[0], "ypass_inv": chosen[1], "ypass_fwd": chosen[2],
-                    "zpass_fwd": chosen[3], "xrow_fused": chosen[4]}
-    traffic, traffic_src = counter_traffic(dom["name"], chosen_named)
+    blob = counter_blob(COUNTER_PROFILE)
+    mix = 1 if args.full_steps else nprof   # a call of full steps has the counters' full-step bytes
+    traffic, traffic_src = counter_launch_traffic(blob, dom["name"], chosen_named, mix)
+    if args.full_steps and blob and dom["name"] in blob.get("launches", {}):
+        traffic = blob["launches"][dom["name"]]["full"]["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
-                "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch,
-                "traffic_source": traffic_src}
-    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
-    full_gbs = step_bytes / (wall_full / args.steps) / 1e9 if wall_full else None
-    dram = counter_step_traffic(chosen_named)
+                "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": round(bytes_per_launch),
+                "launches_timed": dom["launches"], "traffic_source": traffic_src}
+    dram, dram_src = counter_step_dram(blob, chosen_named, 1 if args.full_steps else args.steps)
     kernels = {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4),
                            "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
                            if k["bytes"] > 0 else None} for k in prof}
@@ -420,11 +546,11 @@ def main():
         M._lib.check(M.lib().mhd_bench_copy(nbytes, 20, C.byref(ms)))
         copy_gbs = round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)
         log(f"bench: reference copy kernel {copy_gbs} GB/s (read+write)")
+    del p
 
     large = None
     if not args.no_large:
         log("bench: 1024^3 Float32 MHD RK4")
-        del p
         large = large_bench(M)
         log(f"bench: 1024^3 {large}")
 
@@ -448,8 +574,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("bench: CPU baseline (oracle, literal structure)")
-        cpu = cpu_baseline(fields, n, args.cpu_steps) if fields is not None else None
+        log("bench: CPU baseline (CPU restatement of the reference path, literal structure)")
+        cpu = cpu_baseline(tuple(int(v) for v in args.cpu_sizes.split(",")))
 
     out = {
         "metric": f"MHD RK4 steps/s at {n}^3 (Float32)",
@@ -465,20 +591,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"incompressible MHD {n}^3 Float32 RK4, nu=eta=1e-3, CFL 0.3 per step, "
-                               "seeded solenoidal k^4 exp(-2(k/4)^2) fields |k|<=16",
+                               "seeded solenoidal k^4 exp(-2(k/4)^2) fields |k|<=16"
+                               + (", every step full" if args.full_steps else ""),
                    "grid": n, "precision": "Float32", "stepper": "RK4", "parallelism": "single GPU"},
         "roofline": roofline,
-        "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                          "bytes_basis": "fixed 3-pass model of the step (MHD RK4: 438 V_c + 120 V_r), "
-                                         "independent of the implementation",
-                          "executed_alg_bytes_per_step": round(exec_bytes),
-                          "executed_alg_GBps": round(exec_bytes / (ms_step * 1e-3) / 1e9, 1),
-                          "dram_bytes_per_step": dram[0], "dram_source": dram[1]},
-        "every_step_full": {"value": round(args.steps / wall_full, 3),
-                            "ms_per_step": round(wall_full * 1e3 / args.steps, 4),
-                            "model_bytes_per_step": step_bytes, "achieved": round(full_gbs, 1),
-                            "frac": round(full_gbs / HBM_PEAK_GBS, 4)} if wall_full else None,
+        "step_roofline": step_roofline(step_bytes, args.steps, ms_step, exec_call, ms_full, exec_full, dram,
+                                       dram_src),
         "kernels": kernels,
         "copy_kernel_GBps": copy_gbs,
         "variants": chosen_named,
