@@ -218,6 +218,24 @@ int32_t mhd_spectra(mhd_handle* h, double* out, int32_t nbins_cap, int32_t* nbin
  * field0 .. field0 + ncomp - 1 (1 to 3 components of u or of b): modes with k1 <= |k| <= k2
  * kept, back to real space; ncomp real fields of nx*ny*nz T to host_out. */
 int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, double k1, double k2, void* host_out);
+/* DivVCorrection!(prob) (field = 0) / DivBCorrection!(prob) (field = 1, MHD)
+ * src/Solver/VPSolver.jl:60-139 (exported by src/MHDFlows.jl:62-63): remove the gradient part
+ * of u (b) in sol, phi = -i (k . F) k^-2, F -= i k phi (T arithmetic, k^-2 = 0 at k = 0), then
+ * refresh the three vars fields from all modes (ldiv! of the corrected spectrum). */
+int32_t mhd_div_correction(mhd_handle* h, int32_t field);
+/* Velocity structure functions src/utils/TurbStatTool.jl: SFV(R) = sum_c SFC(v_c)(R) with
+ * SFC(V) = 2 (mean(V) - CF(V)), CF(V) = fftshift(real(ifft(|fft(V)|^2))) (:64-71), binned over
+ * the lags R of the fftshifted cube with the reference's index arithmetic (idx = i - nx/2 at
+ * the 1-based fftshifted position i, so idx = lag + 1).  mode 0: SF2 1D (:88-117), bin
+ * round(|(idx, jdx, kdx)|) = 1 .. nbins, nbins = 2 ceil(|(nx/2, ny/2, nz/2)|).  mode 1:
+ * SFr_Global (:119-160, MHD, nonzero mean field): bin (round(|R cos theta|), round(|R sin
+ * theta|)) with theta the angle to (mean(bx), mean(by), mean(bz)) in T, an nrow x nrow matrix
+ * (nrow = ceil(|(nx/2, ny/2, nz/2)|) + 1) stored column-major as Julia's SFVr[rpar, rperp].
+ * out[0, nbins) receives sum / count per bin (NaN for an empty bin, as the reference's 0/0),
+ * out[nbins, 2 nbins) the counts; out == NULL only reports *nbins and *nrow.  Sums in Float64
+ * (the reference sums SFV of T values in Float64 too). */
+int32_t mhd_structure_function(mhd_handle* h, int32_t mode, double* out, int32_t cap, int32_t* nbins,
+                               int32_t* nrow);
 
 /* ---- checkpoint / restart ------------------------------------------------------ */
 /* savefile(prob, file_number; file_path_and_name) src/integrator.jl:259-288: write the real

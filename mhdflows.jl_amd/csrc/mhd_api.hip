@@ -327,6 +327,24 @@ int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, do
   return h->e->scale_decomposition(field0, ncomp, k1, k2, host_out);
 }
 
+int32_t mhd_div_correction(mhd_handle* h, int32_t field) {
+  MHD_TRY(check_handle(h));
+  return h->e->div_correction(field);
+}
+
+int32_t mhd_structure_function(mhd_handle* h, int32_t mode, double* out, int32_t cap, int32_t* nbins,
+                               int32_t* nrow) {
+  MHD_TRY(check_handle(h));
+  if (mode != 0 && mode != 1) return fail(E_INVALID, "mode must be 0 (SF2 1D) or 1 (SFr_Global)");
+  int nb = 0, nbx = 0;
+  h->e->sf_shape(mode, &nb, &nbx);
+  if (nbins) *nbins = nb;
+  if (nrow) *nrow = nbx;
+  if (!out) return 0;
+  if (cap < 2 * nb) return fail(E_INVALID, "out holds fewer than 2 * nbins values");
+  return h->e->structure_function(mode, out);
+}
+
 int32_t mhd_savefile(mhd_handle* h, const char* path, int32_t exact) {
   MHD_TRY(check_handle(h));
   return save_checkpoint(h->e, 0, path, exact);

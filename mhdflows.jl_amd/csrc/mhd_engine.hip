@@ -1232,6 +1232,105 @@ class Engine : public EngineBase {
     return 0;
   }
 
+  // DivVCorrection! (which = 0) / DivBCorrection! (which = 1), src/Solver/VPSolver.jl:60-139:
+  // project the field triple of sol on k . F = 0, then the triple's vars = C2R of all its modes
+  int div_correction(int which) override {
+    if (nr_ != 1) return fail(E_INVALID, "the divergence corrections act on single-run problems");
+    if (which != 0 && which != 1) return fail(E_INVALID, "field must be 0 (velocity) or 1 (magnetic)");
+    if (which == 1 && phys != PHYS_MHD) return fail(E_INVALID, "DivBCorrection! needs the magnetic field");
+    const int f0 = 3 * which;
+    cx<T>* sol = (cx<T>*)rbh[0].buf[B_SOL] + (int64_t)f0 * g.nc;
+    MHD_TRY(timed("div_correction", 0, [&] { return launch_div_correction<T>(g, sol, st); }));
+    MHD_TRY(colpass("zpass_inv", 2, 1, B_SOL, B_WORK, f0, 0, 3, false));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, 3, false));
+    RowArgs ra = row_args();
+    ra.in_sel = B_WORK;
+    ra.in_f0 = 0;
+    ra.in_ncols = g.nkr;
+    ra.real_sel = 0;
+    ra.real_out_f0 = f0;
+    ra.write_vars = 1;
+    MHD_TRY(xrow("xrow_c2r", ra, ROW_C2R, 3, 0));
+    MHD_TRY(vars_maxima());
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  // Structure functions of the velocity (src/utils/TurbStatTool.jl): mode 0 SF2 1D
+  // (:88-117, bins of round(|R|)), mode 1 SFr_Global (:119-160, bins of |R| parallel and
+  // perpendicular to the mean magnetic field).  Output: normalized bin values (sum / count,
+  // NaN for empty bins as 0/0), Julia column-major for mode 1, then the counts.
+  int sf_shape(int mode, int* nb, int* nbx) const override {
+    const double R = std::sqrt((double)(g.nx / 2) * (g.nx / 2) + (double)(g.ny / 2) * (g.ny / 2) +
+                               (double)(g.nz / 2) * (g.nz / 2));
+    const int Rc = (int)std::ceil(R);
+    if (mode == 0) {
+      *nb = 2 * Rc;
+      *nbx = *nb;
+    } else {
+      *nbx = Rc + 1;
+      *nb = (Rc + 1) * (Rc + 1);
+    }
+    return 0;
+  }
+
+  int structure_function(int mode, double* host_out) override {
+    if (nr_ != 1) return fail(E_INVALID, "structure functions act on single-run problems");
+    if (mode != 0 && mode != 1) return fail(E_INVALID, "mode must be 0 (SF2 1D) or 1 (SFr_Global)");
+    if (mode == 1 && phys != PHYS_MHD) return fail(E_INVALID, "SFr_Global bins against the mean magnetic field");
+    int nb = 0, nbx = 0;
+    sf_shape(mode, &nb, &nbx);
+    const size_t total = 2 * (size_t)nb + 3;
+    if (total > spec_cap_) {
+      MHD_HIP(hipStreamSynchronize(st));
+      if (spec_) (void)hipFree(spec_);
+      spec_ = nullptr;
+      spec_cap_ = 0;
+      MHD_HIP(hipMalloc(&spec_, sizeof(double) * total));
+      spec_cap_ = total;
+    }
+    double* hist = spec_;
+    double* means = spec_ + 2 * (size_t)nb;
+    double mb[3] = {0, 0, 0};
+    if (mode == 1) {  // mean(bx), mean(by), mean(bz) in T
+      MHD_TRY(r2c_fields(3));
+      MHD_TRY(timed("sf_power", 0, [&] { return launch_sf_power<T>(g, rbh[0].buf[B_WORK], means, st); }));
+      MHD_HIP(hipMemcpyAsync(mb, means, sizeof(mb), hipMemcpyDeviceToHost, st));
+      MHD_HIP(hipStreamSynchronize(st));
+      if (mb[0] == 0.0 && mb[1] == 0.0 && mb[2] == 0.0)
+        return fail(E_INVALID, "SFr_Global needs a nonzero mean magnetic field");
+    }
+    MHD_TRY(r2c_fields(0));
+    MHD_TRY(timed("sf_power", 0, [&] { return launch_sf_power<T>(g, rbh[0].buf[B_WORK], means, st); }));
+    double mu[3];
+    MHD_HIP(hipMemcpyAsync(mu, means, sizeof(mu), hipMemcpyDeviceToHost, st));
+    // CF = C2R of the summed power (work field 0; fields 1 and 2 ride along unused)
+    MHD_TRY(colpass("zpass_inv", 2, 1, B_WORK, B_WORK, 0, 0, 3, false));
+    MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, 3, false));
+    RowArgs ra = row_args();
+    ra.in_sel = B_WORK;
+    ra.in_f0 = 0;
+    ra.in_ncols = g.nkr;
+    ra.real_sel = 1;
+    ra.real_out_f0 = 0;
+    ra.write_vars = 1;
+    MHD_TRY(xrow("xrow_c2r", ra, ROW_C2R, 3, 0));
+    MHD_HIP(hipStreamSynchronize(st));
+    const double summean2 = 2.0 * ((double)(T)mu[0] + (double)(T)mu[1] + (double)(T)mu[2]);
+    MHD_HIP(hipMemsetAsync(hist, 0, sizeof(double) * 2 * nb, st));
+    MHD_TRY(timed("sf_bin", 0, [&] {
+      return launch_sf_bin<T>(g, rbh[0].rscr, summean2, mb, hist, nb, nbx, mode == 1, st);
+    }));
+    std::vector<double> h(2 * (size_t)nb);
+    MHD_HIP(hipMemcpyAsync(h.data(), hist, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; ++i) {
+      host_out[i] = h[i] / h[nb + i];
+      host_out[nb + i] = h[nb + i];
+    }
+    return 0;
+  }
+
   // ---- forcing -------------------------------------------------------------------
   // Drop every calcF! module of a run (the reference holds one calcF! per problem, so setting
   // a module replaces the previous one).  The captured step is rebuilt on the next step.

@@ -81,6 +81,9 @@ class EngineBase {
   virtual int spectrum_bins() const = 0;
   virtual int spectra(double* host_out, int nbins) = 0;
   virtual int scale_decomposition(int f0, int ncomp, double k1, double k2, void* host_out) = 0;
+  virtual int div_correction(int which) = 0;
+  virtual int sf_shape(int mode, int* nb, int* nbx) const = 0;
+  virtual int structure_function(int mode, double* host_out) = 0;
   virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;
   virtual int set_forcing_field(int run, const void* const* f3) = 0;
   virtual int set_forcing_nd(int run, double P, const void* const* f3) = 0;

@@ -1195,6 +1195,111 @@ band_kernel(Geom G, cx<T>* __restrict__ work, int ncomp, double k1, double k2) {
   for (int c = 0; c < ncomp; ++c) work[(int64_t)c * G.wnc + idx] = cx<T>{T(0), T(0)};
 }
 
+// DivVCorrection! / DivBCorrection! (src/Solver/VPSolver.jl:60-139) on three compact spectral
+// fields: phi = -i (k . F) k^-2, F -= i k phi, in T in the reference's operation order.
+template <class T>
+__global__ void __launch_bounds__(256)
+div_correction_kernel(Geom G, cx<T>* __restrict__ f) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= G.nc) return;
+  const int kx = (int)(q % G.nkr);
+  const int64_t rest = q / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  const ModeK<T> k = mode_k<T>(G, kx, l, m);
+  cx<T> a = f[q], b = f[G.nc + q], c = f[2 * G.nc + q];
+  // sum = kx a + ky b + kz c; phi = -im * sum; phi = phi * invKrsq
+  const cx<T> sum{(k.kx * a.re + k.ky * b.re) + k.kz * c.re, (k.kx * a.im + k.ky * b.im) + k.kz * c.im};
+  const cx<T> phi{sum.im * k.invK, -sum.re * k.invK};
+  // F -= (im k) phi = (-k phi.im, k phi.re)
+  a = cx<T>{a.re + k.kx * phi.im, a.im - k.kx * phi.re};
+  b = cx<T>{b.re + k.ky * phi.im, b.im - k.ky * phi.re};
+  c = cx<T>{c.re + k.kz * phi.im, c.im - k.kz * phi.re};
+  f[q] = a;
+  f[G.nc + q] = b;
+  f[2 * G.nc + q] = c;
+}
+
+// Structure functions (src/utils/TurbStatTool.jl:67-160): SFC(V) = 2 (mean(V) - CF(V)) with
+// CF(V) = real(ifft(|fft(V)|^2)), summed over the three components.  The power of the three
+// transformed components goes into work field 0 (|X_c|^2 in T, summed in T: the transform is
+// linear, so one inverse transform of the sum gives the sum of the CFs), and mean(V_c) =
+// X_c(0) / N into means[c] (T values held in Float64).
+template <class T>
+__global__ void __launch_bounds__(256)
+sf_power_kernel(Geom G, cx<T>* __restrict__ work, double* __restrict__ means) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= G.nc) return;
+  const int kx = (int)(i % G.nkr);
+  const int64_t rest = i / G.nkr;
+  const int l = (int)(rest % G.ny), m = (int)(rest / G.ny);
+  const int64_t idx = (int64_t)m * G.wps + (int64_t)l * G.wrs + kx;
+  T p = T(0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const cx<T> v = work[(int64_t)c * G.wnc + idx];
+    p += v.re * v.re + v.im * v.im;
+    if (i == 0) means[c] = (double)(v.re * T(G.inv_n));
+  }
+  work[idx] = cx<T>{p, T(0)};
+}
+
+// Binning of SFV(R) = sum_c 2 (mean(V_c) - CF_c(R)) over the lags R of the fftshifted cube, as
+// the reference indexes it: the element at fftshifted 1-based position i has idx = i - nx/2,
+// i.e. idx = s + 1 for the signed lag s in [-nx/2, nx/2) (likewise jdx, kdx).
+//   radial (SF2 1D):    kk = round(|(idx, jdx, kdx)|), kk > 0 -> bin kk - 1 of nb
+//   global (SFr_Global): kk = |(idx, jdx, kdx)| > 0, theta = angle(mean b, (idx, jdx, kdx)) in T,
+//                         bin (round(|kk cos theta|), round(|kk sin theta|)), row length nbx
+// hist[0, nb) sums SFV, hist[nb, 2 nb) counts.  Rounding is half to even (Julia round).
+template <class T, bool GLOBAL>
+__global__ void __launch_bounds__(256)
+sf_bin_kernel(Geom G, const T* __restrict__ cf, double summean2, double mbx, double mby, double mbz,
+              double* __restrict__ hist, int nb, int nbx) {
+  extern __shared__ double lh[];  // radial: 2 nb doubles
+  const int t = threadIdx.x;
+  if (!GLOBAL) {
+    for (int i = t; i < 2 * nb; i += 256) lh[i] = 0.0;
+    __syncthreads();
+  }
+  const T bx = T(mbx), by = T(mby), bz = T(mbz);
+  const T sb = bx * bx + by * by + bz * bz;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + t; p < G.nr; p += (int64_t)gridDim.x * 256) {
+    const int px = (int)(p % G.nx);
+    const int64_t rest = p / G.nx;
+    const int py = (int)(rest % G.ny), pz = (int)(rest / G.ny);
+    const int idx = (px < G.nx / 2 ? px : px - G.nx) + 1;
+    const int jdx = (py < G.ny / 2 ? py : py - G.ny) + 1;
+    const int kdx = (pz < G.nz / 2 ? pz : pz - G.nz) + 1;
+    const int64_t ss = (int64_t)idx * idx + (int64_t)jdx * jdx + (int64_t)kdx * kdx;
+    if (ss == 0) continue;
+    const double kk = sqrt((double)ss);
+    const T v = T(summean2) - T(2) * cf[p];
+    int bin;
+    if (!GLOBAL) {
+      bin = (int)rint(kk) - 1;
+    } else {
+      T x = (bx * T(idx) + by * T(jdx) + bz * T(kdx)) / sqrt(sb * T(ss));
+      x = x > T(1) ? T(1) : (x < T(-1) ? T(-1) : x);
+      const T th = acos(x);
+      const int rpar = (int)rint(fabs(kk * (double)cos(th)));
+      const int rperp = (int)rint(fabs(kk * (double)sin(th)));
+      bin = rperp * nbx + rpar;
+    }
+    if (bin < 0 || bin >= nb) continue;
+    if (GLOBAL) {
+      atomicAdd(&hist[bin], (double)v);
+      atomicAdd(&hist[nb + bin], 1.0);
+    } else {
+      atomicAdd(&lh[bin], (double)v);
+      atomicAdd(&lh[nb + bin], 1.0);
+    }
+  }
+  if (!GLOBAL) {
+    __syncthreads();
+    for (int i = t; i < 2 * nb; i += 256)
+      if (lh[i] != 0.0) atomicAdd(&hist[i], lh[i]);
+  }
+}
+
 // scatter sparse half-spectrum entries (kx, l, m) of nf fields into x (already zeroed)
 template <class T>
 __global__ void __launch_bounds__(256)

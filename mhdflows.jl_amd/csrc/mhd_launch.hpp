@@ -94,6 +94,13 @@ hipError_t launch_shell(const Geom& g, const cx<T>* work, int ncomp, double* out
 template <class T>
 hipError_t launch_band(const Geom& g, cx<T>* work, int ncomp, double k1, double k2, hipStream_t s);
 template <class T>
+hipError_t launch_div_correction(const Geom& g, cx<T>* f, hipStream_t s);
+template <class T>
+hipError_t launch_sf_power(const Geom& g, cx<T>* work, double* means, hipStream_t s);
+template <class T>
+hipError_t launch_sf_bin(const Geom& g, const T* cf, double summean2, const double* mb, double* hist, int nb,
+                         int nbx, bool global, hipStream_t s);
+template <class T>
 hipError_t launch_nd_amp(const RunBuf<T>* runs, int nruns, int64_t nblocks, double dV, int gate, hipStream_t s);
 
 enum SmallKernel { SK_CFL = 0, SK_REDUCE = 1, SK_ADVANCE = 2, SK_VARSMAX = 3, SK_PRUNE = 4 };

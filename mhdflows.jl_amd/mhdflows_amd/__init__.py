@@ -5,6 +5,9 @@ libmhdflows_amd.so (include/mhdflows_amd.h)."""
 from ._lib import MHDError, lib  # noqa: F401
 from .problem import (  # noqa: F401
     ScaleDecomposition,
+    SFr,
+    DivVCorrection,
+    DivBCorrection,
     spectra,
     spectralline,
     Restart,
@@ -46,5 +49,5 @@ __all__ = [
     "MHDcalcN_advection", "HDcalcN_advection", "UiUpdate", "BiUpdate", "ProbDiagnostic",
     "Diagnostics", "set_forcing", "get_sol_modes", "N97ForceDriving", "NDForceDriving",
     "A99ForceDriving", "SetUpN97", "SetUpND", "SetUpFk", "SetUpForcingField", "savefile", "Restart",
-    "readMHDFlows", "read_checkpoint", "spectra", "spectralline", "ScaleDecomposition", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
+    "readMHDFlows", "read_checkpoint", "spectra", "spectralline", "ScaleDecomposition", "SFr", "DivVCorrection", "DivBCorrection", "refresh_vars", "MHDError", "MHDFlowsProblem", "nothingfunction",
 ]

@@ -84,6 +84,8 @@ SIGNATURES = {
     "mhd_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
     "mhd_spectra": (_I, [_P, _P, _I, C.POINTER(_I)]),
     "mhd_scale_decomposition": (_I, [_P, _I, _I, C.c_double, C.c_double, _P]),
+    "mhd_structure_function": (_I, [_P, _I, _P, _I, _P, _P]),
+    "mhd_div_correction": (_I, [_P, _I]),
     "mhd_savefile": (_I, [_P, C.c_char_p, _I]),
     "mhd_restart": (_I, [_P, C.c_char_p, _I]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),

@@ -292,10 +292,25 @@ def stepforward(prob, nsteps=1):
 
 
 def getCFL(prob, t_diff=None, Coef=0.3):
-    """``getCFL!`` (``src/integrator.jl:158-198``); sets and returns ``clock.dt``."""
+    """``getCFL!`` (``src/integrator.jl:158-198``); sets and returns ``clock.dt``.  The library
+    bounds dt by the diffusive limit of ``Coef``; a ``t_diff`` passed here bounds it as well."""
     dt = C.c_double()
     check(lib().mhd_cfl_dt(prob._h, float(Coef), C.byref(dt)))
+    if t_diff is not None and t_diff < dt.value:
+        prob.clock.dt = t_diff
+        return float(t_diff)
     return dt.value
+
+
+def diffusion_dt(prob, CFL_Coef):
+    """The diffusive time-step bound ``t_diff`` of ``TimeIntegrator!`` (``src/integrator.jl:60-72``)."""
+    pr, g = prob.params, prob.grid
+    if prob.flag.b:
+        vi, nv = max(pr.nu, pr.eta), max(pr.nnu, pr.neta)
+    else:
+        vi, nv = pr.nu, pr.nnu
+    dl = min(g.Lx / g.nx, g.Ly / g.ny, g.Lz / g.nz)
+    return CFL_Coef * dl ** nv / vi if nv > 1 else CFL_Coef * dl ** 2 / vi
 
 
 def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothingfunction, diags=(),
@@ -304,8 +319,9 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
     """``TimeIntegrator!`` (``src/integrator.jl:31-156``): at most ``N0+1`` steps while ``t <= t0``.
 
     The hot loop runs on the device (dt from ``getCFL!`` computed there).  Diagnostics
-    callables in ``diags`` and ``prob.usr_func`` hooks are called once per step and force
-    a per-step loop on the host; without them the loop has no host synchronisation.
+    callables in ``diags``, ``prob.usr_func`` hooks, a user ``CFL_function(prob, t_diff,
+    Coef=...)`` (which sets ``prob.clock.dt``) and saving force a per-step loop on the host;
+    without them the loop has no host synchronisation.
     """
     file_path_and_name = ""
     if save:
@@ -314,12 +330,11 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
         file_path_and_name = save_loc + filename                   # integrator.jl:44-51
         savefile(prob, file_number, file_path_and_name=file_path_and_name)
         file_number += 1
-    if CFL_function is not nothingfunction:
-        if usr_dt > 0.0:
-            raise MHDError(-1, "User define both CFL_function and usr_dt")
-        raise MHDError(-2, "user CFL functions run on the host and are not supported on the device loop")
+    user_cfl = CFL_function is not nothingfunction
+    if user_cfl and usr_dt > 0.0:                                 # CFL_Init, integrator.jl:200-206
+        raise MHDError(-1, "User define both CFL_function and usr_dt")
     hooks = [f for f in prob.usr_func if f is not nothingfunction]
-    if not diags and not hooks and not save:
+    if not diags and not hooks and not save and not user_cfl:
         steps = C.c_int64()
         check(lib().mhd_integrate(prob._h, float(t0), int(N0), float(usr_dt), float(CFL_Coef), C.byref(steps)))
         return steps.value
@@ -329,8 +344,11 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
     prob.clock.step = 0
     if usr_dt != 0.0:
         prob.clock.dt = usr_dt
+    t_diff = diffusion_dt(prob, CFL_Coef)
     while N0 >= prob.clock.step and t0 >= prob.clock.t:
-        if usr_dt == 0.0:
+        if user_cfl:
+            CFL_function(prob, t_diff, Coef=CFL_Coef)             # sets prob.clock.dt
+        elif usr_dt == 0.0:
             getCFL(prob, Coef=CFL_Coef)
         stepforward(prob, 1)
         taken += 1
@@ -515,6 +533,38 @@ def ScaleDecomposition(prob, fields=("ux", "uy", "uz"), kf=(1, 5)):
     check(lib().mhd_scale_decomposition(prob._h, idx[0], len(idx), float(min(kf)), float(max(kf)),
                                         out.ctypes.data_as(C.c_void_p)))
     return tuple(out)
+
+
+def DivVCorrection(prob):
+    """``DivVCorrection!(prob)`` (``src/Solver/VPSolver.jl:100-139``): remove the gradient part
+    of u from sol (phi = -i (k . u) / k^2, u -= i k phi) and refresh vars.ux..uz."""
+    check(lib().mhd_div_correction(prob._h, 0))
+
+
+def DivBCorrection(prob):
+    """``DivBCorrection!(prob)`` (``src/Solver/VPSolver.jl:60-98``): the same for b (MHD)."""
+    check(lib().mhd_div_correction(prob._h, 1))
+
+
+def SFr(prob, mode="1D"):
+    """Velocity structure functions of the current vars (``src/utils/TurbStatTool.jl``):
+    ``mode="1D"`` is ``SF₂1D(ux, uy, uz)`` (:88-117), a vector of 2 ceil(|(nx/2, ny/2, nz/2)|)
+    radial bins; ``mode="Global"`` is ``SFr(...; mode="Global")`` = ``SFr_Global`` (:119-160), the
+    (nrow, nrow) matrix ``SFVr[rpar, rperp]`` against the mean magnetic field.  Empty bins are NaN
+    (the reference's 0/0).  Returns (values, counts)."""
+    m = {"1D": 0, "Global": 1}.get(mode)
+    if m is None:
+        raise MHDError(-1, f"mode must be '1D' or 'Global', not {mode!r}")
+    nb, nrow = C.c_int32(), C.c_int32()
+    check(lib().mhd_structure_function(prob._h, m, None, 0, C.byref(nb), C.byref(nrow)))
+    out = np.empty(2 * nb.value, np.float64)
+    check(lib().mhd_structure_function(prob._h, m, out.ctypes.data_as(C.c_void_p), out.size, C.byref(nb),
+                                       C.byref(nrow)))
+    vals, cnt = out[:nb.value], out[nb.value:]
+    if m == 1:   # column-major [rpar, rperp] -> numpy [rpar, rperp]
+        vals = vals.reshape(nrow.value, nrow.value).T
+        cnt = cnt.reshape(nrow.value, nrow.value).T
+    return vals, cnt
 
 
 # ---- checkpoint / restart (src/integrator.jl:208-288, src/utils/IC.jl:245-257) ----

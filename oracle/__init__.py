@@ -48,6 +48,11 @@ How each part is pinned (checked by ``tests/test_oracle_*.py``)
   and independence of its counter-based draws.  A99's random stream itself is parity unpinned:
   the reference draws unseeded rand(), so the build defines the draws (oracle and device share
   them) and only the distribution and construction can be compared.
+* Structure functions (``analysis.cf``, ``sf2_1d``, ``sfr_global``, TurbStatTool.jl): CF against a
+  brute-force correlation sum, the binning against the same sums taken over signed lags with the
+  reference's idx = lag + 1, and the global binning against an axis-aligned mean field.
+* Divergence corrections (``vp``): the Helmholtz split (gradient removed, solenoidal part kept,
+  k . F = 0 after) and idempotence.
 * LSRK54: order conditions of the stability polynomial (Taylor series through z^4).
 * Alias index table: the kept ranges listed for the FourierFlows 1/3 rule.
 

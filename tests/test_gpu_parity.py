@@ -230,6 +230,41 @@ def test_cfl_time_integrator():
     assert rel_l2(p2.sol, o2.sol) < 1e-12
 
 
+def test_host_hooks_run_once_per_step():
+    """increment!(diags) and prob.usr_func run after every stepforward! (integrator.jl:115,131-133);
+    the per-step host loop they force gives the device loop's state."""
+    say("host hooks")
+    _, a = pair(32, T=np.float64, nu=1e-3, eta=1e-3, kmax=6.0)
+    _, b = pair(32, T=np.float64, nu=1e-3, eta=1e-3, kmax=6.0)
+    seen, energies = [], []
+    a.usr_func.append(lambda prob: seen.append(prob.clock.step))
+    taken = M.TimeIntegrator(a, math.inf, 4, CFL_Coef=0.3, diags=[lambda prob: energies.append(
+        M.Diagnostics(prob)["E_kin"])])
+    M.TimeIntegrator(b, math.inf, 4, CFL_Coef=0.3)
+    assert taken == 5 and seen == [1, 2, 3, 4, 5] and len(energies) == 5
+    assert energies[-1] == M.Diagnostics(a)["E_kin"]
+    assert math.isclose(a.clock.t, b.clock.t, rel_tol=1e-14)
+    assert rel_l2(a.sol, b.sol) < 1e-13
+
+
+def test_user_cfl_function():
+    """A user CFL_function(prob, t_diff; Coef) sets dt before every step (integrator.jl:53-54,107)."""
+    say("user CFL")
+    _, a = pair(16, T=np.float64, nu=2e-3, eta=1e-3, kmax=4.0)
+    _, b = pair(16, T=np.float64, nu=2e-3, eta=1e-3, kmax=4.0)
+    calls = []
+
+    def my_cfl(prob, t_diff, Coef=0.3):
+        calls.append((t_diff, Coef))
+        prob.clock.dt = 5e-4
+
+    M.TimeIntegrator(a, math.inf, 3, CFL_Coef=0.2, CFL_function=my_cfl)
+    M.TimeIntegrator(b, math.inf, 3, usr_dt=5e-4)
+    dl = 2 * math.pi / 16
+    assert len(calls) == 4 and all(math.isclose(t, 0.2 * dl * dl / 2e-3) and c == 0.2 for t, c in calls)
+    assert rel_l2(a.sol, b.sol) < 1e-14 and a.clock.step == 4
+
+
 def test_cfl_fp32_matches_oracle():
     o, p = pair(32, T=np.float32, nu=1e-3, eta=1e-3, kmax=6.0)
     time_integrator(o, 1e9, 2, CFL_Coef=0.3)
