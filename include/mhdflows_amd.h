@@ -199,6 +199,15 @@ int32_t mhd_set_forcing_field(mhd_handle* h, const void* const* f3);
  * with u the real fields of the stage input; the sum is accumulated in Float64 (the
  * reference sums in T).  f3: 3 real fields (NULL = zero).  Single-run problems only. */
 int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3);
+/* Volume penalisation (Problem(...; VP_method = true), src/pgen.jl:84; MHDParams_VP /
+ * HDParams_VP src/Structure/MHDParams.jl:1-40): every right-hand side adds VP_UᵢUpdate! and,
+ * for MHD, VP_BᵢUpdate! (src/Solver/VPSolver.jl:21-56): N_a += -sum_j (delta_aj - k_j k_a
+ * k^-2) R2C(chi/eta (u_j - U0_j)) (b, B0 likewise) with eta = clock.dt * 13 / 7, in T.
+ * chi (params.χ), U0[3] (params.U₀x/y/z, set by SetUpProblemIC!'s U₀x..U₀z), B0[3] (MHD):
+ * real fields of nx*ny*nz T; a NULL pointer keeps that field (zero when first enabled).
+ * enable = 0 turns it off.  Single-run problems; not combined with negative damping; the
+ * per-direction updates (mhd_u_update, mhd_b_update) refuse VP problems. */
+int32_t mhd_set_vp(mhd_handle* h, int32_t enable, const void* chi, const void* const* U0, const void* const* B0);
 /* A99ForceDriving! (src/pgen/A99ForceDriving.jl:33-60) with SetUpFk(prob; kf, P, σ²)
  * (:93-127), usr_vars.b = b and usr_vars.A = A.  The reference draws unseeded rand() per
  * mode and call; here the four draws of a mode (phase 1, the complex Φ, phase 2) are

@@ -308,6 +308,10 @@ int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3) {
   if (!f3) return fail(E_INVALID, "f3 is NULL");
   return h->e->set_forcing_nd(0, P, f3);
 }
+int32_t mhd_set_vp(mhd_handle* h, int32_t enable, const void* chi, const void* const* U0, const void* const* B0) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_vp(0, enable, chi, U0, B0);
+}
 int32_t mhd_set_forcing_a99(mhd_handle* h, uint64_t seed, double kf, double P, double sigma2, double b, double A) {
   MHD_TRY(check_handle(h));
   return h->e->set_forcing_a99(0, seed, kf, P, sigma2, b, A);

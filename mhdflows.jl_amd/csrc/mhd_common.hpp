@@ -173,6 +173,9 @@ struct RunBuf {
   uint64_t a99_seed;
   double a99_kf, a99_s2, a99_b, a99_A, a99_amp;  // kf, sigma^2, b, usr A, setup amplitude (all rounded to T)
   double a99_den;       // tanh(b pi / 2) evaluated in T
+  // volume penalisation (MHDParams_VP / HDParams_VP): real fields chi, U0x, U0y, U0z and, for
+  // MHD, B0x, B0y, B0z (nr each), or null
+  const T* vp;
 };
 
 }  // namespace mhd

@@ -108,6 +108,8 @@ class Engine : public EngineBase {
   // calcF! module buffers per run: static forcing spectrum, negative-damping f fields and sums
   std::vector<void*> fdense_bufs, ndf_bufs, ndpart_bufs;
   bool nd_any = false;     // a run has negative damping: MHD row launches form 12 products
+  std::vector<void*> vp_bufs;  // volume penalisation fields per run (chi, U0, B0)
+  bool vp_any = false;     // a run has volume penalisation: row launches form nf more products
   int np_alloc = 0;        // product fields each run's work buffer holds
   int64_t calls_ = 0;      // right-hand sides evaluated by calcN (draw counter of stochastic forcing)
   double* dpart = nullptr;     // FP64 diagnostic partials
@@ -120,7 +122,7 @@ class Engine : public EngineBase {
     for (void* p : allocs) hipFree(p);
     for (void* p : force_tabs)
       if (p) hipFree(p);
-    for (auto* v : {&fdense_bufs, &ndf_bufs, &ndpart_bufs})
+    for (auto* v : {&fdense_bufs, &ndf_bufs, &ndpart_bufs, &vp_bufs})
       for (void* p : *v)
         if (p) hipFree(p);
     if (dpart) hipFree(dpart);
@@ -174,6 +176,7 @@ class Engine : public EngineBase {
     fdense_bufs.assign(nr_, nullptr);
     ndf_bufs.assign(nr_, nullptr);
     ndpart_bufs.assign(nr_, nullptr);
+    vp_bufs.assign(nr_, nullptr);
     np_alloc = np;
     for (int r = 0; r < nr_; ++r) {
       RunBuf<T>& b = rbh[r];
@@ -533,10 +536,12 @@ class Engine : public EngineBase {
     const bool fused_planes = prune && sp.mask_input && sp.self_sel < 0;
     MHD_TRY(colpass("zpass_inv", 2, 1, in_sel, B_WORK, 0, 0, nf, prune, true, fused_planes));
     MHD_TRY(colpass("ypass_inv", 1, 1, B_WORK, B_WORK, 0, 0, nf, prune, true));
-    // negative damping adds three products (f_i u_i) and its sum to the MHD row launch
+    // negative damping adds three products (f_i u_i) and its sum to the MHD row launch;
+    // volume penalisation adds nf products chi/eta (v - V0) (u, and b for MHD) to every one
     const bool nd = nd_any && phys == PHYS_MHD && sp.add_force;
-    const int rvar = nd ? 0 : row_var;
-    const int npf = nd ? 12 : np;
+    const bool vp = vp_any;
+    const int rvar = (nd || vp) ? 0 : row_var;
+    const int npf = nd ? 12 : (vp ? np + nf : np);
     RowArgs ra = row_args(rvar);
     ra.in_sel = B_WORK;
     ra.in_ncols = prune ? g.kx_keep : g.nkr;
@@ -548,8 +553,10 @@ class Engine : public EngineBase {
     const int kout = host_pruned_ ? g.kx_keep : g.nkr;
     // the real fields are stored at the last stage of a full step only
     double bytes = rows * (nf * ra.in_ncols + npf * kout) * sizeof(cx<T>) +
-                   (last && !host_pruned_ ? rows * nf * g.nx * sizeof(T) : 0) + (nd ? rows * 3 * g.nx * sizeof(T) : 0);
-    MHD_TRY(xrow("xrow_fused", ra, phys == PHYS_MHD ? (nd ? ROW_MHD_ND : ROW_MHD) : ROW_HD, nf, bytes, rvar));
+                   (last && !host_pruned_ ? rows * nf * g.nx * sizeof(T) : 0) + (nd ? rows * 3 * g.nx * sizeof(T) : 0) +
+                   (vp ? rows * (1 + nf) * g.nx * sizeof(T) : 0);
+    const int kind = phys == PHYS_MHD ? (nd ? ROW_MHD_ND : (vp ? ROW_MHD_VP : ROW_MHD)) : (vp ? ROW_HD_VP : ROW_HD);
+    MHD_TRY(xrow("xrow_fused", ra, kind, nf, bytes, rvar));
     MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, npf, false, true, false, gate_ != 0));
     MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, npf, false, true, false, gate_ != 0));
     if (nd) {
@@ -592,14 +599,15 @@ class Engine : public EngineBase {
   // extra complex fields the spectral launch reads per mode for forcing (dense spectrum or
   // negative-damping products)
   int forcing_fields() const {
+    int n = 0;
     for (int r = 0; r < nr_; ++r)
-      if (rbh[r].fdense || rbh[r].ndf) return 3;
-    return 0;
+      if (rbh[r].fdense || rbh[r].ndf) n = 3;
+    return n + (vp_any ? nf : 0);
   }
 
   bool any_module() const {
     for (int r = 0; r < nr_; ++r)
-      if (rbh[r].fdense || rbh[r].ndf || rbh[r].a99) return true;
+      if (rbh[r].fdense || rbh[r].ndf || rbh[r].a99 || rbh[r].vp) return true;
     return false;
   }
 
@@ -957,6 +965,8 @@ class Engine : public EngineBase {
 
   int component_update(void* N, int field) override {
     if (nr_ != 1) return fail(E_INVALID, "component updates act on single-run problems");
+    if (vp_any)
+      return fail(E_UNSUPPORTED, "the per-direction updates do not include volume penalisation; use calcN");
     if (field < 0 || field >= nf) return fail(E_INVALID, "direction out of range");
     rbh[0].buf[B_EXT_N] = (cx<T>*)N;
     MHD_TRY(upload_runs());
@@ -1409,6 +1419,7 @@ class Engine : public EngineBase {
   int set_forcing_nd(int run, double P, const void* const* f3) override {
     MHD_TRY(check_run(run));
     if (nr_ != 1) return fail(E_INVALID, "negative damping acts on single-run problems");
+    if (rbh[run].vp) return fail(E_UNSUPPORTED, "negative damping with volume penalisation is not supported");
     MHD_TRY(clear_forcing(run));
     RunBuf<T>& b = rbh[run];
     void* d = nullptr;
@@ -1421,20 +1432,65 @@ class Engine : public EngineBase {
     }
     MHD_TRY(device_alloc(sizeof(double) * (size_t)row_blocks, &d));
     ndpart_bufs[run] = d;
-    if (phys == PHYS_MHD && np_alloc < 12) {  // room for the three f_i u_i products
-      MHD_HIP(hipStreamSynchronize(st));
-      void* w = nullptr;
-      MHD_TRY(device_alloc(sizeof(cx<T>) * (size_t)g.wnc * 12, &w));
-      for (auto& p : allocs)
-        if (p == (void*)b.buf[B_WORK]) p = w;
-      (void)hipFree(b.buf[B_WORK]);
-      b.buf[B_WORK] = (cx<T>*)w;
-      np_alloc = 12;
-    }
+    if (phys == PHYS_MHD) MHD_TRY(grow_work(run, 12));  // room for the three f_i u_i products
     b.ndf = (const T*)ndf_bufs[run];
     b.ndpart = (double*)ndpart_bufs[run];
     b.nd_P = P;
     nd_any = true;
+    return forcing_done();
+  }
+
+  // the work buffer of a run holds at least nfields product fields
+  int grow_work(int run, int nfields) {
+    if (np_alloc >= nfields) return 0;
+    RunBuf<T>& b = rbh[run];
+    MHD_HIP(hipStreamSynchronize(st));
+    void* w = nullptr;
+    MHD_TRY(device_alloc(sizeof(cx<T>) * (size_t)g.wnc * nfields, &w));
+    for (auto& p : allocs)
+      if (p == (void*)b.buf[B_WORK]) p = w;
+    (void)hipFree(b.buf[B_WORK]);
+    b.buf[B_WORK] = (cx<T>*)w;
+    np_alloc = nfields;
+    return 0;
+  }
+
+  // Volume penalisation (VP_method, MHDParams_VP/HDParams_VP): chi, U0 (and B0 for MHD) as real
+  // fields; a NULL pointer leaves that field as it was (zero when first enabled).  enable = 0
+  // turns it off and frees the fields.
+  int set_vp(int run, int enable, const void* chi, const void* const* U0, const void* const* B0) override {
+    MHD_TRY(check_run(run));
+    if (nr_ != 1) return fail(E_INVALID, "volume penalisation acts on single-run problems");
+    RunBuf<T>& b = rbh[run];
+    MHD_HIP(hipStreamSynchronize(st));
+    if (!enable) {
+      if (vp_bufs[run]) (void)hipFree(vp_bufs[run]);
+      vp_bufs[run] = nullptr;
+      b.vp = nullptr;
+      vp_any = false;
+      drop_graph();
+      return forcing_done();
+    }
+    if (b.ndf) return fail(E_UNSUPPORTED, "volume penalisation with negative damping is not supported");
+    const int nfield = 1 + nf;  // chi, then U0 (and B0)
+    if (!vp_bufs[run]) {
+      void* d = nullptr;
+      MHD_TRY(device_alloc(sizeof(T) * nfield * (size_t)g.nr, &d));
+      MHD_HIP(hipMemsetAsync(d, 0, sizeof(T) * nfield * (size_t)g.nr, st));
+      vp_bufs[run] = d;
+    }
+    T* base = (T*)vp_bufs[run];
+    const size_t fb = sizeof(T) * (size_t)g.nr;
+    if (chi) MHD_HIP(hipMemcpyAsync(base, chi, fb, hipMemcpyHostToDevice, st));
+    for (int c = 0; c < 3; ++c) {
+      if (U0 && U0[c]) MHD_HIP(hipMemcpyAsync(base + (1 + c) * (size_t)g.nr, U0[c], fb, hipMemcpyHostToDevice, st));
+      if (phys == PHYS_MHD && B0 && B0[c])
+        MHD_HIP(hipMemcpyAsync(base + (4 + c) * (size_t)g.nr, B0[c], fb, hipMemcpyHostToDevice, st));
+    }
+    MHD_TRY(grow_work(run, np + nf));
+    b.vp = base;
+    vp_any = true;
+    drop_graph();
     return forcing_done();
   }
 

@@ -87,6 +87,7 @@ class EngineBase {
   virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;
   virtual int set_forcing_field(int run, const void* const* f3) = 0;
   virtual int set_forcing_nd(int run, double P, const void* const* f3) = 0;
+  virtual int set_vp(int run, int enable, const void* chi, const void* const* U0, const void* const* B0) = 0;
   virtual int set_forcing_a99(int run, uint64_t seed, double kf, double P, double s2, double b, double A) = 0;
   virtual int time_steps(int nsteps, double* ms) = 0;
   virtual int profile_steps(int nsteps, std::vector<KTime>* out) = 0;

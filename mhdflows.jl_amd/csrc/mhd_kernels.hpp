@@ -136,6 +136,8 @@ enum RowKind {
   ROW_VARS_MHD = 4,   // 6 real rows -> 9 products and the 6 inputs -> r2c (15 rows)
   ROW_VARS_HD = 5,    // 3 real rows -> 6 products and the 3 inputs -> r2c (9 rows)
   ROW_MHD_ND = 6,     // ROW_MHD plus negative damping: 3 more products f_i u_i and sum |u_i^2 f_i|
+  ROW_MHD_VP = 7,     // ROW_MHD plus volume penalisation: 6 more products chi/eta (u_j - U0_j), chi/eta (b_j - B0_j)
+  ROW_HD_VP = 8,      // ROW_HD plus volume penalisation: 3 more products chi/eta (u_j - U0_j)
 };
 
 struct RowArgs {
@@ -167,13 +169,15 @@ template <int NF> struct RowKindInfo<ROW_R2C, NF> { static constexpr int NIN = N
 template <int NF> struct RowKindInfo<ROW_VARS_MHD, NF> { static constexpr int NIN = 6, NPROD = 9, NSELF = 6; };
 template <int NF> struct RowKindInfo<ROW_VARS_HD, NF> { static constexpr int NIN = 3, NPROD = 6, NSELF = 3; };
 template <int NF> struct RowKindInfo<ROW_MHD_ND, NF> { static constexpr int NIN = 6, NPROD = 12, NSELF = 0; };
+template <int NF> struct RowKindInfo<ROW_MHD_VP, NF> { static constexpr int NIN = 6, NPROD = 15, NSELF = 0; };
+template <int NF> struct RowKindInfo<ROW_HD_VP, NF> { static constexpr int NIN = 3, NPROD = 9, NSELF = 0; };
 
 // Real-space products of one point; u = v[0..2], b = v[3..5].
 // MHD: T_ij = b_i b_j - u_i u_j (i<=j: 00,01,02,11,12,22), M_aj = u_a b_j - b_a u_j (01,02,12).
 // HD:  U_ij = u_i u_j.
 template <class T, int KIND>
 __device__ __forceinline__ T product(int p, const T* v) {
-  if constexpr (KIND == ROW_MHD || KIND == ROW_VARS_MHD || KIND == ROW_MHD_ND) {
+  if constexpr (KIND == ROW_MHD || KIND == ROW_VARS_MHD || KIND == ROW_MHD_ND || KIND == ROW_MHD_VP) {
     switch (p) {
       case 0: return v[3] * v[3] - v[0] * v[0];
       case 1: return v[3] * v[4] - v[0] * v[1];
@@ -266,8 +270,12 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   using Rfwd = typename RReverse<Rinv>::type;
   constexpr int R0 = RFirst<Rinv>::value;   // first inverse radix == last forward radix
   constexpr int RL = RLast<Rinv>::value;    // last inverse radix == first forward radix
-  constexpr bool C2R_IN = (KIND == ROW_MHD || KIND == ROW_HD || KIND == ROW_C2R || KIND == ROW_MHD_ND);
+  constexpr bool C2R_IN = (KIND == ROW_MHD || KIND == ROW_HD || KIND == ROW_C2R || KIND == ROW_MHD_ND ||
+                           KIND == ROW_MHD_VP || KIND == ROW_HD_VP);
   constexpr bool ND = KIND == ROW_MHD_ND;
+  // volume penalisation: the advective products come first, then one product per field
+  constexpr bool VP = KIND == ROW_MHD_VP || KIND == ROW_HD_VP;
+  constexpr int NADV = KIND == ROW_MHD_VP ? 9 : (KIND == ROW_HD_VP ? 6 : 0);
   constexpr int NIN = RowKindInfo<KIND, NF>::NIN;
   constexpr int NPROD = RowKindInfo<KIND, NF>::NPROD;
 
@@ -455,10 +463,32 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
     // time); Float64 rows keep it rolled, as their six resident fields leave no room for
     // the longer live ranges of the unrolled form.
     constexpr int UNR = sizeof(T) == 4 ? NOUT : 1;
+    // VP_UᵢUpdate! / VP_BᵢUpdate! (src/Solver/VPSolver.jl:21-56): chi / eta with
+    // eta = clock.dt * 13 / 7, all in T as the reference's broadcasts evaluate it
+    const T eta = VP ? T(rb.clk->dt) * T(13) / T(7) : T(1);
 #pragma unroll UNR
     for (int p = 0; p < NOUT; ++p) {
       cx<T> z[E];
-      if (ND && p >= 9) {
+      if (VP && p >= NADV) {
+        // chi/eta * (v_c - W_c) with W = U0 (c < 3) or B0 (c >= 3): rb.vp fields 0 (chi) and
+        // 1 + c, read at the thread's samples for each product (no registers held across them)
+        const gptr<const T> cr = gmem<const T>(rb.vp) + row * NX;
+#pragma unroll
+        for (int c = 0; c < NIN; ++c)
+          if (c == p - NADV) {
+            const gptr<const T> wr = cr + (int64_t)(1 + c) * G.nr;
+#pragma unroll
+            for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+              for (int r = 0; r < RL; ++r) {
+                const int j = tr + i * TX + r * (M / RL);
+                const int e = i * RL + r;
+                const cx<T> x = row_ok ? ldg((gptr<const cx<T>>)(cr + 2 * j)) : cx<T>{T(0), T(0)};
+                const cx<T> w = row_ok ? ldg((gptr<const cx<T>>)(wr + 2 * j)) : cx<T>{T(0), T(0)};
+                z[e] = cx<T>{(x.re / eta) * (v[c][e].re - w.re), (x.im / eta) * (v[c][e].im - w.im)};
+              }
+          }
+      } else if (ND && p >= 9) {
         // F_i = f_i u_i (NegativeDamping.jl:39); c is matched against p so that the register
         // arrays are indexed by constants only
 #pragma unroll
@@ -807,6 +837,26 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
     cx<T> R[NF];
     const int64_t qp = ms[j] * plane_stride(G, a.prod_sel) + ls[j] * row_stride(G, a.prod_sel) + kx;
     rhs_at<T, PHYS>(G, rb, a, qp, k, sH, R);
+    if (MODS && rb.vp) {
+      // VP_UᵢUpdate! / VP_BᵢUpdate! (src/Solver/VPSolver.jl:21-56): N_a += -(delta_aj - k_j k_a k^-2) P_j
+      // for the penalisation products P of u (and b), which follow the advective ones
+      constexpr int NADV = PHYS == PHYS_MHD ? 9 : 6;
+      const gptr<const cx<T>> Pv = gmem<const cx<T>>(rb.buf[a.prod_sel]);
+      const int64_t pfs = field_stride(G, a.prod_sel);
+      const T kv[3] = {k.kx, k.ky, k.kz};
+#pragma unroll
+      for (int g = 0; g < NF / 3; ++g)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T kak2 = kv[c] * k.invK;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const T coef = T(c == j ? 1 : 0) - kv[j] * kak2;
+            const cx<T> P = ldst<true, T>(Pv + (int64_t)(NADV + 3 * g + j) * pfs + qp);
+            R[3 * g + c] = R[3 * g + c] + cx<T>{-coef * P.re, -coef * P.im};
+          }
+        }
+    }
     // addforcing! (src/pgen.jl:231-234) follows the MHD advection; HD discards it (:176-178)
     if (PHYS == PHYS_MHD && a.add_force) {
       if (rb.force) {  // seeded large-scale shell forcing

@@ -26,13 +26,14 @@ Method map (reference â†’ this file):
   spectralline, ScaleDecomposition        src/utils/MHDAnalysis.jl:17-82,236-255 (of prob.vars, on the device)
   SFâ‚‚1D / SFr(...; "Global")              src/utils/TurbStatTool.jl:88-160 â†’ SFr(prob, mode)
   DivVCorrection!, DivBCorrection!        src/Solver/VPSolver.jl:60-139
+  VP_method, params.Ï‡/Uâ‚€/Bâ‚€               src/Solver/VPSolver.jl:21-56 â†’ Problem(...; VP_method), SetUpVP!
 """
 module MHDFlowsMI355X
 
 using FourierFlows
 import MHDFlows
 
-export MI355X, set_step_pruning!, autotune!, setsol!, SetUpN97!, SetUpND!, SetUpFk, spectra, SFr
+export MI355X, set_step_pruning!, autotune!, setsol!, SetUpN97!, SetUpND!, SetUpFk, spectra, SFr, SetUpVP!
 
 const libmhd = joinpath(@__DIR__, "..", "mhdflows_amd", "libmhdflows_amd.so")
 
@@ -122,7 +123,7 @@ struct MI355XParams
     calcF!::Function
 end
 
-"`prob.flag`: b (MHD), e (EMHD), vp, c (compressible), s (shear) â€” only b can be set here."
+"`prob.flag`: b (MHD), e (EMHD), vp, c (compressible), s (shear) â€” b and vp can be set here."
 struct MI355XFlag
     b::Bool; e::Bool; vp::Bool; c::Bool; s::Bool
 end
@@ -154,7 +155,8 @@ const DEVICE_FORCING = (N97ForceDriving!, NDForceDriving!, A99ForceDriving!)
 
 function MHDFlows.Problem(::MI355X; nx=64, ny=nx, nz=nx, Lx=2Ï€, Ly=Lx, Lz=Lx, Î½=0.0, nÎ½=0, Î·=0.0, nÎ·=0,
                           B_field=false, stepper="RK4", T=Float32, dt=0.0, aliased_fraction=1/3,
-                          calcF=MHDFlows.nothingfunction, usr_func=[], device=0, hbm_budget=0, kwargs...)
+                          calcF=MHDFlows.nothingfunction, usr_func=[], VP_method=false, device=0, hbm_budget=0,
+                          kwargs...)
     # aliased_fraction and nÎ· are accepted and ignored, as in src/pgen.jl:91,107 and :114-116
     if calcF !== MHDFlows.nothingfunction && !(calcF in DEVICE_FORCING)
         # addforcing! calls calcF! inside every right-hand side (src/pgen.jl:231-234): a Julia
@@ -173,7 +175,10 @@ function MHDFlows.Problem(::MI355X; nx=64, ny=nx, nz=nx, Lx=2Ï€, Ly=Lx, Lz=Lx, Î
                          pts(Lx, dx, nx), pts(Ly, dy, ny), pts(Lz, dz, nz))
     nf = B_field ? 6 : 3
     params = MI355XParams(Î½, Î·, nÎ½, nÎ·, 1, 2, 3, 4, 5, 6, calcF)
-    flag = MI355XFlag(B_field, false, false, false, false)
+    flag = MI355XFlag(B_field, false, VP_method, false, false)
+    # MHDParams_VP / HDParams_VP: chi, U0 (and B0) start as zero fields (datastructure.jl:79-95)
+    VP_method && check(ccall((:mhd_set_vp, libmhd), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ptr{Ptr{Cvoid}},
+                                                          Ptr{Ptr{Cvoid}}), h[], 1, C_NULL, C_NULL, C_NULL))
     prob = MI355XProblem{T}(h[], cfg, grid, MI355XClock(h[]), MI355XVars{T}(h[], nx, ny, nz, nf),
                             params, flag, nf, Function[f for f in usr_func])
     finalizer(p -> ccall((:mhd_destroy, libmhd), Int32, (Ptr{Cvoid},), getfield(p, :h)), prob)
@@ -181,13 +186,31 @@ function MHDFlows.Problem(::MI355X; nx=64, ny=nx, nz=nx, Lx=2Ï€, Ly=Lx, Lz=Lx, Î
 end
 
 function MHDFlows.SetUpProblemIC!(prob::MI355XProblem{T}; ux=[], uy=[], uz=[], bx=[], by=[], bz=[],
-                                  kwargs...) where T
+                                  Uâ‚€x=[], Uâ‚€y=[], Uâ‚€z=[], Bâ‚€x=[], Bâ‚€y=[], Bâ‚€z=[], kwargs...) where T
+    if !all(isempty, (Uâ‚€x, Uâ‚€y, Uâ‚€z, Bâ‚€x, Bâ‚€y, Bâ‚€z))          # IC.jl:93-105
+        prob.flag.vp || error("Uâ‚€/Bâ‚€ fields need a VP_method problem")
+        SetUpVP!(prob; Uâ‚€x=Uâ‚€x, Uâ‚€y=Uâ‚€y, Uâ‚€z=Uâ‚€z, Bâ‚€x=Bâ‚€x, Bâ‚€y=Bâ‚€y, Bâ‚€z=Bâ‚€z)
+    end
     fields = prob.nf == 6 ? (ux, uy, uz, bx, by, bz) : (ux, uy, uz)
     arrays = [f == [] ? nothing : convert(Array{T,3}, f) for f in fields]
     ptrs = Ptr{Cvoid}[a === nothing ? C_NULL : pointer(a) for a in arrays]
     GC.@preserve arrays check(ccall((:mhd_set_ic_real, libmhd), Int32, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}),
                                     prob.h, ptrs))
     return nothing
+end
+
+"""Volume-penalisation fields of a VP_method problem: `Ï‡` (what the reference's user copies into
+`prob.params.Ï‡`, e.g. `Cylindrical_Mask_Function(prob.grid)`), `Uâ‚€x..z`, `Bâ‚€x..z`; a field left
+empty keeps its value.  Every right-hand side then adds VP_Uáµ¢Update!/VP_Báµ¢Update!."""
+function SetUpVP!(prob::MI355XProblem{T}; Ï‡=[], Uâ‚€x=[], Uâ‚€y=[], Uâ‚€z=[], Bâ‚€x=[], Bâ‚€y=[], Bâ‚€z=[]) where T
+    conv(a) = isempty(a) ? nothing : convert(Array{T,3}, a)
+    arrays = [conv(a) for a in (Ï‡, Uâ‚€x, Uâ‚€y, Uâ‚€z, Bâ‚€x, Bâ‚€y, Bâ‚€z)]
+    p(a) = a === nothing ? C_NULL : pointer(a)
+    U = Ptr{Cvoid}[p(a) for a in arrays[2:4]]
+    B = Ptr{Cvoid}[p(a) for a in arrays[5:7]]
+    GC.@preserve arrays check(ccall((:mhd_set_vp, libmhd), Int32,
+                                    (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}),
+                                    prob.h, 1, p(arrays[1]), U, B))
 end
 
 "Copy `prob.sol` (Complex{T}[nkr, ny, nz, nf], the reference layout) to the host."
@@ -234,7 +257,8 @@ function MHDFlows.TimeIntegrator!(prob::MI355XProblem, tâ‚€::Number, Nâ‚€::Int; 
         error("User define both CFL_function and usr_dt")
     end
     hooks = [f for f in prob.usr_func if f != MHDFlows.nothingfunction]
-    if isempty(diags) && isempty(hooks) && !save && !user_cfl
+    vp = prob.flag.vp
+    if isempty(diags) && isempty(hooks) && !save && !user_cfl && !vp
         # the whole loop on the device: dt from getCFL! there, no host synchronisation
         steps = Ref{Int64}(0)
         check(ccall((:mhd_integrate, libmhd), Int32,
@@ -247,6 +271,10 @@ function MHDFlows.TimeIntegrator!(prob::MI355XProblem, tâ‚€::Number, Nâ‚€::Int; 
     t_next_save = prob.clock.t + dump_dt
     prob.clock.step = 0
     usr_dt != 0.0 && (prob.clock.dt = usr_dt)
+    if vp                                                        # :84-87
+        MHDFlows.DivVCorrection!(prob)
+        prob.flag.b && MHDFlows.DivBCorrection!(prob)
+    end
     while Nâ‚€ >= prob.clock.step && tâ‚€ >= prob.clock.t
         if user_cfl
             CFL_function(prob, t_diff; Coef=CFL_Coef)           # sets prob.clock.dt
@@ -256,6 +284,10 @@ function MHDFlows.TimeIntegrator!(prob::MI355XProblem, tâ‚€::Number, Nâ‚€::Int; 
         MHDFlows.stepforward!(prob)
         for d in diags                                           # increment!(diags)
             FourierFlows.increment!(d)
+        end
+        if vp                                                    # :118-121
+            MHDFlows.DivVCorrection!(prob)
+            prob.flag.b && MHDFlows.DivBCorrection!(prob)
         end
         for foo! in hooks                                        # usr_func
             foo!(prob)

@@ -86,6 +86,7 @@ SIGNATURES = {
     "mhd_scale_decomposition": (_I, [_P, _I, _I, C.c_double, C.c_double, _P]),
     "mhd_structure_function": (_I, [_P, _I, _P, _I, _P, _P]),
     "mhd_div_correction": (_I, [_P, _I]),
+    "mhd_set_vp": (_I, [_P, _I, _P, _P, _P]),
     "mhd_savefile": (_I, [_P, C.c_char_p, _I]),
     "mhd_restart": (_I, [_P, C.c_char_p, _I]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),

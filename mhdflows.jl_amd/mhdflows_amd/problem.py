@@ -215,7 +215,8 @@ def Problem(dev="MI355X", **kw):
     eta = float(_pick(kw, ["η", "eta"], 0.0))
     _pick(kw, ["nη", "neta"], 0)
     B = bool(_pick(kw, ["B_field"], False))
-    for flag in ("EMHD", "Compressibility", "Shear", "VP_method", "Dye_Module"):
+    vp = bool(_pick(kw, ["VP_method"], False))
+    for flag in ("EMHD", "Compressibility", "Shear", "Dye_Module"):
         if _pick(kw, [flag], False):
             raise MHDError(-2, f"{flag} is not part of the MI355X hot path")
     stepper = _pick(kw, ["stepper"], "RK4")
@@ -244,11 +245,28 @@ def Problem(dev="MI355X", **kw):
                    physics=L.MHD_PHYS_MHD if B else L.MHD_PHYS_HD,
                    stepper=L.MHD_RK4 if stepper == "RK4" else L.MHD_LSRK54,
                    device=device, stream=None, hbm_budget=budget, dt=dt)
-    return MHDFlowsProblem(cfg, T, calcF)
+    prob = MHDFlowsProblem(cfg, T, calcF)
+    if vp:   # MHDParams_VP / HDParams_VP: chi, U0 (and B0) start as zero fields (datastructure.jl:79-95)
+        check(lib().mhd_set_vp(prob._h, 1, None, None, None))
+        prob.flag.vp = True
+    return prob
 
 
 def SetUpProblemIC(prob, ux=None, uy=None, uz=None, bx=None, by=None, bz=None, **kw):
-    """``SetUpProblemIC!`` (``src/utils/IC.jl:41-109``): real fields in ``[z][y][x]`` order."""
+    """``SetUpProblemIC!`` (``src/utils/IC.jl:41-109``): real fields in ``[z][y][x]`` order.
+    ``U₀x``..``U₀z`` and ``B₀x``..``B₀z`` (ASCII ``U0x``.. ``B0z``) set the volume-penalisation
+    fields of a ``VP_method`` problem (:93-105)."""
+    vpk = {}
+    for name, alias in (("U₀x", "U0x"), ("U₀y", "U0y"), ("U₀z", "U0z"), ("B₀x", "B0x"), ("B₀y", "B0y"), ("B₀z", "B0z")):
+        v = kw.pop(name, kw.pop(alias, None))
+        if v is not None and not (isinstance(v, (list, tuple)) and len(v) == 0):
+            vpk[alias] = v
+    if kw:
+        raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
+    if vpk:
+        if not prob.flag.vp:
+            raise MHDError(-1, "U₀/B₀ fields need a VP_method problem")
+        SetUpVP(prob, **vpk)
     g = prob.grid
     arrs = [ux, uy, uz] + ([bx, by, bz] if prob.nf == 6 else [])
     keep = []
@@ -263,6 +281,46 @@ def SetUpProblemIC(prob, ux=None, uy=None, uz=None, bx=None, by=None, bz=None, *
         keep.append(a)
         ptrs[f] = a.ctypes.data
     check(lib().mhd_set_ic_real(prob._h, ptrs))
+
+
+def SetUpVP(prob, chi=None, U0x=None, U0y=None, U0z=None, B0x=None, B0y=None, B0z=None, **kw):
+    """Volume-penalisation fields of a ``VP_method`` problem: ``params.χ`` (what the reference's
+    user copies in, e.g. ``Cylindrical_Mask_Function``) and ``params.U₀x..z``, ``B₀x..z``
+    (``src/Structure/MHDParams.jl:1-40``), real ``[z][y][x]`` arrays; a field not given keeps its
+    value.  Every right-hand side then adds VP_UᵢUpdate!/VP_BᵢUpdate! (src/Solver/VPSolver.jl:21-56)."""
+    chi = kw.pop("χ", chi)
+    if kw:
+        raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
+    if not prob.flag.vp:
+        raise MHDError(-1, "SetUpVP needs a problem created with VP_method=True")
+    g = prob.grid
+    keep = []
+
+    def ptr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=prob.T), (g.nz, g.ny, g.nx)))
+        keep.append(a)
+        return a.ctypes.data
+
+    U = (C.c_void_p * 3)(*[ptr(a) for a in (U0x, U0y, U0z)])
+    B = (C.c_void_p * 3)(*[ptr(a) for a in (B0x, B0y, B0z)])
+    check(lib().mhd_set_vp(prob._h, 1, ptr(chi), U, B))
+
+
+def Cylindrical_Mask_Function(grid, R2=0.82 * math.pi, R1=0.0, **kw):
+    """``Cylindrical_Mask_Function(grid; R₂, R₁)`` (``src/utils/IC.jl:15-27``): 0 where
+    ``R₂ >= sqrt(x^2 + y^2) >= R₁`` (the fluid), 1 elsewhere (the solid), as a ``[z][y][x]`` array
+    of the grid's type (the reference's BitArray copied into ``params.χ``)."""
+    R2 = kw.pop("R₂", R2)
+    R1 = kw.pop("R₁", R1)
+    if kw:
+        raise MHDError(-1, f"unknown keyword(s) {sorted(kw)}")
+    x = grid.x[None, None, :]
+    y = grid.y[None, :, None]
+    R = np.sqrt(x * x + y * y)          # in T, compared with the Float64 radii
+    fluid = (R2 >= R.astype(np.float64)) & (R.astype(np.float64) >= R1)
+    return np.broadcast_to(np.where(fluid, 0, 1), (grid.nz, grid.ny, grid.nx)).astype(grid.T)
 
 
 def SetUpProblemIC_modes(prob, idx, vals):
@@ -334,7 +392,8 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
     if user_cfl and usr_dt > 0.0:                                 # CFL_Init, integrator.jl:200-206
         raise MHDError(-1, "User define both CFL_function and usr_dt")
     hooks = [f for f in prob.usr_func if f is not nothingfunction]
-    if not diags and not hooks and not save and not user_cfl:
+    vp = prob.flag.vp
+    if not diags and not hooks and not save and not user_cfl and not vp:
         steps = C.c_int64()
         check(lib().mhd_integrate(prob._h, float(t0), int(N0), float(usr_dt), float(CFL_Coef), C.byref(steps)))
         return steps.value
@@ -345,6 +404,10 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
     if usr_dt != 0.0:
         prob.clock.dt = usr_dt
     t_diff = diffusion_dt(prob, CFL_Coef)
+    if vp:                                                        # :84-87
+        DivVCorrection(prob)
+        if prob.flag.b:
+            DivBCorrection(prob)
     while N0 >= prob.clock.step and t0 >= prob.clock.t:
         if user_cfl:
             CFL_function(prob, t_diff, Coef=CFL_Coef)             # sets prob.clock.dt
@@ -354,6 +417,10 @@ def TimeIntegrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25, CFL_function=nothing
         taken += 1
         for d in diags:                                           # increment!(diags)
             d(prob)
+        if vp:                                                    # :118-121
+            DivVCorrection(prob)
+            if prob.flag.b:
+                DivBCorrection(prob)
         for f in hooks:                                           # usr_func
             f(prob)
         if save and prob.clock.t >= t_next_save:                  # :136-141

@@ -16,10 +16,22 @@ def time_integrator(prob, t0, N0, usr_dt=0.0, CFL_Coef=0.25):
     usr = usr_dt != 0.0
     if usr:
         prob.dt = prob.grid.T.type(usr_dt)                     # :80-82
+    if prob.vp is not None:                                    # :84-87
+        _vp_corrections(prob)
     while N0 >= prob.step and t0 >= prob.t:
         if not usr:
             prob.dt = cfl_dt(prob, td, CFL_Coef)               # :106-109
         step(prob)                                             # :112-113
+        if prob.vp is not None:                                # :118-121
+            _vp_corrections(prob)
+
+
+def _vp_corrections(prob):
+    """DivVCorrection!, then DivBCorrection! for MHD (the VP branch of TimeIntegrator!)."""
+    from .vp import div_correction
+    div_correction(prob, 0)
+    if prob.physics == "MHD":
+        div_correction(prob, 1)
 
 
 def run_steps(prob, nsteps):

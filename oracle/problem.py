@@ -35,6 +35,7 @@ class OracleProblem:
         self.step = 0
         self.forcing = forcing
         self.force_counter = 1 << 40   # draws of a right-hand side outside a step (calcN calls)
+        self.vp = None                 # volume penalisation: {"chi", "U0", "B0"} real fields (oracle.vp)
 
     def set_ic(self, ux=None, uy=None, uz=None, bx=None, by=None, bz=None):
         """``SetUpProblemIC!`` (``src/utils/IC.jl:41-109``): copy real fields to vars, R2C into sol."""

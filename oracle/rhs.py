@@ -65,6 +65,7 @@ def mhd_u_update_literal(N, sol, t, prob, a):
                 if i != j:
                     coef = (1j * k[j]) * (_delta(a, i) - (ka * k[i]) * invK)
                     Na += coef * nonlinh1                       # :79
+    vp_update(Na, prob, a, 0)                                   # :85-88
     uh = r2c(u[a])                                              # :91-93
     _promoted_diffusion(Na, g.Krsq, prob.nu, uh)                # :94
     if prob.nnu > 1:                                            # :97-99
@@ -84,6 +85,7 @@ def mhd_b_update_literal(N, sol, t, prob, a):
             nonlin1 = u[a] * b[j] - b[a] * u[j]                 # :150
             nonlinh1 = r2c(nonlin1)                             # :152
             Nb += (1j * k[j]) * nonlinh1                        # :155
+    vp_update(Nb, prob, a, 3)                                   # :160-163
     bh = r2c(b[a])                                              # :166-167
     _promoted_diffusion(Nb, g.Krsq, prob.eta, bh)               # :168
     if prob.neta > 1:                                           # :171-173
@@ -118,6 +120,7 @@ def hd_u_update_literal(N, sol, t, prob, a):
                 if i != j:
                     coef = (-1j * k[j]) * (_delta(a, i) - (ka * k[i]) * invK)
                     Na += coef * uuh                            # :70
+    vp_update(Na, prob, a, 0)                                   # HDSolver.jl:76-79
     uh = r2c(u[a])                                              # :82-84
     _promoted_diffusion(Na, g.Krsq, prob.nu, uh)                # :85
     if prob.nnu > 1:                                            # :88-90
@@ -163,6 +166,7 @@ def mhd_advection_fused(N, sol, t, prob):
     sp = hermitian_project(sol)
     for a in range(3):
         Na = (D[a] - k[a] * (kD * g.invKrsq)).astype(g.C)
+        vp_update(Na, prob, a, 0)
         _promoted_diffusion(Na, g.Krsq, prob.nu, sp[a])
         if prob.nnu > 1:
             _promoted_diffusion(Na, g.Krsq, prob.nu, sp[a], power=prob.nnu)
@@ -173,6 +177,7 @@ def mhd_advection_fused(N, sol, t, prob):
         Mh[(j, a)] = -Mh[(a, j)]
     for a in range(3):
         Nb = sum((1j * k[j]) * Mh[(a, j)] for j in range(3) if j != a).astype(g.C)
+        vp_update(Nb, prob, a, 3)
         _promoted_diffusion(Nb, g.Krsq, prob.eta, sp[3 + a])
         if prob.neta > 1:
             _promoted_diffusion(Nb, g.Krsq, prob.eta, sp[3 + a], power=prob.neta)
@@ -197,6 +202,7 @@ def hd_advection_fused(N, sol, t, prob):
     sp = hermitian_project(sol[0:3])
     for a in range(3):
         Na = (D[a] - k[a] * (kD * g.invKrsq)).astype(g.C)
+        vp_update(Na, prob, a, 0)
         _promoted_diffusion(Na, g.Krsq, prob.nu, sp[a])
         if prob.nnu > 1:
             _promoted_diffusion(Na, g.Krsq, prob.nu, sp[a], power=prob.nnu)
@@ -232,3 +238,6 @@ def add_forcing(N, sol, t, prob):
         return
     Fh = prob.forcing.spectrum(prob.grid)  # (3, nz, ny, nkr), already scaled by N
     N[0:3] += Fh.astype(prob.grid.C)
+
+
+from .vp import vp_update  # noqa: E402  (oracle.vp imports the transforms above)
