@@ -206,7 +206,10 @@ int32_t mhd_set_forcing_nd(mhd_handle* h, double P, const void* const* f3);
  * chi (params.χ), U0[3] (params.U₀x/y/z, set by SetUpProblemIC!'s U₀x..U₀z), B0[3] (MHD):
  * real fields of nx*ny*nz T; a NULL pointer keeps that field (zero when first enabled).
  * enable = 0 turns it off.  Single-run problems; not combined with negative damping; the
- * per-direction updates (mhd_u_update, mhd_b_update) refuse VP problems. */
+ * per-direction updates (mhd_u_update, mhd_b_update) refuse VP problems.  TimeIntegrator!'s
+ * DivVCorrection!/DivBCorrection! around every step (src/integrator.jl:84-87,118-121) belong
+ * to the host loop (mhd_div_correction between mhd_step calls); mhd_step and mhd_integrate
+ * do not apply them. */
 int32_t mhd_set_vp(mhd_handle* h, int32_t enable, const void* chi, const void* const* U0, const void* const* B0);
 /* A99ForceDriving! (src/pgen/A99ForceDriving.jl:33-60) with SetUpFk(prob; kf, P, σ²)
  * (:93-127), usr_vars.b = b and usr_vars.A = A.  The reference draws unseeded rand() per
