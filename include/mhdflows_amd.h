@@ -230,6 +230,14 @@ int32_t mhd_spectra(mhd_handle* h, double* out, int32_t nbins_cap, int32_t* nbin
  * field0 .. field0 + ncomp - 1 (1 to 3 components of u or of b): modes with k1 <= |k| <= k2
  * kept, back to real space; ncomp real fields of nx*ny*nz T to host_out. */
 int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, double k1, double k2, void* host_out);
+/* Vector calculus of host real fields (nx*ny*nz T each, the problem's grid) on the device, in T
+ * as the reference evaluates them: op 3/4/5 = ∂i(B1, "x"/"y"/"z") (src/utils/VectorCalculus.jl:
+ * 101-123, 1 in, 1 out), 6 = LaplaceSolver(B) (:155-168: Φ = C2R(R2C(B) k^-2), the reference's
+ * sign, 1 in, 1 out), 7 = Curl(B1, B2, B3) (:27-50, 3 in, 3 out), 8 = Div(B1, B2, B3) (:70-92,
+ * 3 in, 1 out), 9 = VectorPotential(B1, B2, B3) (src/utils/MHDAnalysis.jl:156-180, Coulomb
+ * gauge, 3 in, 3 out).  A NULL input is a zero field.  Uses the problem's work space (not
+ * during a call in flight on the same handle). */
+int32_t mhd_field_op(mhd_handle* h, int32_t op, const void* const* in, void* const* out);
 /* DivVCorrection!(prob) (field = 0) / DivBCorrection!(prob) (field = 1, MHD)
  * src/Solver/VPSolver.jl:60-139 (exported by src/MHDFlows.jl:62-63): remove the gradient part
  * of u (b) in sol, phi = -i (k . F) k^-2, F -= i k phi (T arithmetic, k^-2 = 0 at k = 0), then

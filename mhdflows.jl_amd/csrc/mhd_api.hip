@@ -331,6 +331,10 @@ int32_t mhd_scale_decomposition(mhd_handle* h, int32_t field0, int32_t ncomp, do
   return h->e->scale_decomposition(field0, ncomp, k1, k2, host_out);
 }
 
+int32_t mhd_field_op(mhd_handle* h, int32_t op, const void* const* in, void* const* out) {
+  MHD_TRY(check_handle(h));
+  return h->e->field_op(op, in, out);
+}
 int32_t mhd_div_correction(mhd_handle* h, int32_t field) {
   MHD_TRY(check_handle(h));
   return h->e->div_correction(field);

@@ -1242,6 +1242,41 @@ class Engine : public EngineBase {
     return 0;
   }
 
+  // Vector calculus of host real fields on the device (VectorCalculus.jl, MHDAnalysis.jl):
+  // upload nin fields, R2C, the spectral op in T, C2R of nout fields, download
+  int field_op(int op, const void* const* in, void* const* out) override {
+    if (nr_ != 1) return fail(E_INVALID, "field operations act on single-run problems");
+    int nin = 0, nout = 0;
+    switch (op) {
+      case OP_CURL_T: case OP_VECPOT_T: nin = 3; nout = 3; break;
+      case OP_DIV_T: nin = 3; nout = 1; break;
+      case OP_DX: case OP_DY: case OP_DZ: case OP_INVLAP: nin = 1; nout = 1; break;
+      default: return fail(E_INVALID, "unknown field operation");
+    }
+    if (!in || !out) return fail(E_INVALID, "NULL field arrays");
+    for (int c = 0; c < nout; ++c)
+      if (!out[c]) return fail(E_INVALID, "NULL output field");
+    RunBuf<T>& b = rbh[0];
+    const size_t fb = sizeof(T) * (size_t)g.nr;
+    for (int c = 0; c < 3; ++c) {
+      if (c < nin && in[c]) MHD_HIP(hipMemcpyAsync(b.rscr + (size_t)c * g.nr, in[c], fb, hipMemcpyHostToDevice, st));
+      else MHD_HIP(hipMemsetAsync(b.rscr + (size_t)c * g.nr, 0, fb, st));
+    }
+    RowArgs ra = row_args();
+    ra.in_f0 = 0;
+    ra.real_sel = 1;
+    ra.out2_sel = B_WORK;
+    ra.out2_f0 = 0;
+    MHD_TRY(xrow("xrow_r2c", ra, ROW_R2C, 3, 0));
+    MHD_TRY(colpass("ypass_fwd", 1, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, 3, false));
+    MHD_TRY(diag_field(op, nout == 1 ? 1 : 3));
+    for (int c = 0; c < nout; ++c)
+      MHD_HIP(hipMemcpyAsync(out[c], b.rscr + (size_t)c * g.nr, fb, hipMemcpyDeviceToHost, st));
+    MHD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+
   // DivVCorrection! (which = 0) / DivBCorrection! (which = 1), src/Solver/VPSolver.jl:60-139:
   // project the field triple of sol on k . F = 0, then the triple's vars = C2R of all its modes
   int div_correction(int which) override {

@@ -82,6 +82,7 @@ class EngineBase {
   virtual int spectra(double* host_out, int nbins) = 0;
   virtual int scale_decomposition(int f0, int ncomp, double k1, double k2, void* host_out) = 0;
   virtual int div_correction(int which) = 0;
+  virtual int field_op(int op, const void* const* in, void* const* out) = 0;
   virtual int sf_shape(int mode, int* nb, int* nbx) const = 0;
   virtual int structure_function(int mode, double* host_out) = 0;
   virtual int set_forcing(int run, uint64_t seed, double kf, double amp) = 0;

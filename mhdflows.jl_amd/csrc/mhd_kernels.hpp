@@ -1157,7 +1157,11 @@ dot_max_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t nr, int
 // Spectral operators for the diagnostics on a 3-component half spectrum X
 // (fields 0..2 of src) written to dst fields 0..2 (curl / vector potential) or
 // field 0 (divergence).  Wavenumbers in FP64.
-enum DiagOp { OP_CURL = 0, OP_VECPOT = 1, OP_DIV = 2 };
+// OP_CURL / OP_VECPOT / OP_DIV: diagnostics in Float64 wavenumbers.  OP_DX..OP_DZ, OP_INVLAP and
+// OP_CURL_T / OP_DIV_T / OP_VECPOT_T: the reference's VectorCalculus.jl / MHDAnalysis.jl
+// functions, evaluated in T with the grid's wavenumber arrays in T as those functions do.
+enum DiagOp { OP_CURL = 0, OP_VECPOT = 1, OP_DIV = 2, OP_DX = 3, OP_DY = 4, OP_DZ = 5, OP_INVLAP = 6,
+              OP_CURL_T = 7, OP_DIV_T = 8, OP_VECPOT_T = 9 };
 // src and dst are fields of the work buffer (padded planes: field stride wnc, plane stride wps).
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -1171,6 +1175,31 @@ diag_spectral_kernel(Geom G, const cx<T>* __restrict__ src, cx<T>* __restrict__ 
   const double k0 = kx * G.multx, k1 = signed_index(l, G.ny) * G.multy, k2 = signed_index(m, G.nz) * G.multz;
   const cx<T> a0 = src[idx], a1 = src[G.wnc + idx], a2 = src[2 * G.wnc + idx];
   auto ik = [](double kk, cx<T> z) { return cx<T>{T(-kk * (double)z.im), T(kk * (double)z.re)}; };
+  if (op >= OP_DX) {
+    const ModeK<T> q = mode_k<T>(G, kx, l, m);
+    // im * z: (-z.im, z.re) exactly
+    auto im = [](cx<T> z) { return cx<T>{-z.im, z.re}; };
+    auto sc = [](T c, cx<T> z) { return cx<T>{c * z.re, c * z.im}; };
+    if (op == OP_DX || op == OP_DY || op == OP_DZ) {   // ∂i: im*k_i*B1h (VectorCalculus.jl:101-123)
+      const T kk = op == OP_DX ? q.kx : (op == OP_DY ? q.ky : q.kz);
+      dst[idx] = im(sc(kk, a0));
+    } else if (op == OP_INVLAP) {                      // LaplaceSolver: Bh *= k^-2 (:155-168)
+      dst[idx] = sc(q.invK, a0);
+    } else if (op == OP_DIV_T) {                       // Div: im*(kx*B1h + ky*B2h + kz*B3h) (:81-92)
+      dst[idx] = im(sc(q.kx, a0) + sc(q.ky, a1) + sc(q.kz, a2));
+    } else {                                            // Curl (:27-50) / VectorPotential (MHDAnalysis.jl:156-180)
+      cx<T> c0 = im(sc(q.ky, a2) - sc(q.kz, a1));
+      cx<T> c1 = im(sc(q.kz, a0) - sc(q.kx, a2));
+      cx<T> c2 = im(sc(q.kx, a1) - sc(q.ky, a0));
+      if (op == OP_VECPOT_T) {
+        c0 = sc(q.invK, c0);
+        c1 = sc(q.invK, c1);
+        c2 = sc(q.invK, c2);
+      }
+      dst[idx] = c0; dst[G.wnc + idx] = c1; dst[2 * G.wnc + idx] = c2;
+    }
+    return;
+  }
   if (op == OP_DIV) {
     const cx<T> d = ik(k0, a0) + ik(k1, a1) + ik(k2, a2);
     dst[idx] = d;

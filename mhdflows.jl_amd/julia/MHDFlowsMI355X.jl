@@ -27,6 +27,8 @@ Method map (reference → this file):
   SF₂1D / SFr(...; "Global")              src/utils/TurbStatTool.jl:88-160 → SFr(prob, mode)
   DivVCorrection!, DivBCorrection!        src/Solver/VPSolver.jl:60-139
   VP_method, params.χ/U₀/B₀               src/Solver/VPSolver.jl:21-56 → Problem(...; VP_method), SetUpVP!
+  Curl, Div, ∂i, LaplaceSolver, VectorPotential  src/utils/VectorCalculus.jl, MHDAnalysis.jl:129-180
+                                          → the same names with the problem first (its grid, on the device)
 """
 module MHDFlowsMI355X
 
@@ -391,6 +393,26 @@ function MHDFlows.ScaleDecomposition(prob::MI355XProblem{T}, fields::Tuple; kf=[
                 prob.h, f0, length(fields), minimum(kf), maximum(kf), out))
     return Tuple(out[:, :, :, c] for c in 1:length(fields))
 end
+
+# Vector calculus of host fields on the device (src/utils/VectorCalculus.jl, MHDAnalysis.jl):
+# the problem's grid and precision; op codes of mhd_field_op
+function fieldop(prob::MI355XProblem{T}, op::Integer, fields, nout::Integer) where T
+    g = prob.grid
+    ins = [convert(Array{T,3}, f) for f in fields]
+    outs = [Array{T,3}(undef, g.nx, g.ny, g.nz) for _ in 1:nout]
+    pin = Ptr{Cvoid}[pointer(a) for a in ins]
+    pout = Ptr{Cvoid}[pointer(a) for a in outs]
+    GC.@preserve ins outs check(ccall((:mhd_field_op, libmhd), Int32, (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}},
+                                      Ptr{Ptr{Cvoid}}), prob.h, op, pin, pout))
+    return outs
+end
+MHDFlows.Curl(prob::MI355XProblem, B1, B2, B3) = Tuple(fieldop(prob, 7, (B1, B2, B3), 3))
+MHDFlows.Div(prob::MI355XProblem, B1, B2, B3) = fieldop(prob, 8, (B1, B2, B3), 1)[1]
+MHDFlows.∂i(prob::MI355XProblem, B1, direction) =
+    fieldop(prob, direction == "x" ? 3 : direction == "y" ? 4 : direction == "z" ? 5 :
+                  error("Wrong driection declared"), (B1,), 1)[1]
+MHDFlows.LaplaceSolver(prob::MI355XProblem, B) = fieldop(prob, 6, (B,), 1)[1]
+MHDFlows.VectorPotential(prob::MI355XProblem, B1, B2, B3) = Tuple(fieldop(prob, 9, (B1, B2, B3), 3))
 
 # DivVCorrection! / DivBCorrection! (src/Solver/VPSolver.jl:60-139, exported by MHDFlows)
 MHDFlows.DivVCorrection!(prob::MI355XProblem) =

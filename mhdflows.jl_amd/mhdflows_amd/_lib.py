@@ -87,6 +87,7 @@ SIGNATURES = {
     "mhd_structure_function": (_I, [_P, _I, _P, _I, _P, _P]),
     "mhd_div_correction": (_I, [_P, _I]),
     "mhd_set_vp": (_I, [_P, _I, _P, _P, _P]),
+    "mhd_field_op": (_I, [_P, _I, _P, _P]),
     "mhd_savefile": (_I, [_P, C.c_char_p, _I]),
     "mhd_restart": (_I, [_P, C.c_char_p, _I]),
     "mhd_autotune": (_I, [_P, _I, C.c_char_p, _I]),

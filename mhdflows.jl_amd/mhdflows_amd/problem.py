@@ -602,6 +602,83 @@ def ScaleDecomposition(prob, fields=("ux", "uy", "uz"), kf=(1, 5)):
     return tuple(out)
 
 
+# ---- vector calculus on the device (src/utils/VectorCalculus.jl, src/utils/MHDAnalysis.jl) ----
+_OP = {"x": 3, "y": 4, "z": 5, "laplace": 6, "curl": 7, "div": 8, "vecpot": 9}
+
+
+def _field_op(prob, op, fields, nout):
+    g = prob.grid
+    keep = []
+    ins = (C.c_void_p * 3)()
+    for c, a in enumerate(fields):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=prob.T), (g.nz, g.ny, g.nx)))
+        keep.append(a)
+        ins[c] = a.ctypes.data
+    outs = [np.empty((g.nz, g.ny, g.nx), prob.T) for _ in range(nout)]
+    optr = (C.c_void_p * 3)(*[o.ctypes.data for o in outs])
+    check(lib().mhd_field_op(prob._h, op, ins, optr))
+    return outs
+
+
+def Curl(prob, B1, B2, B3):
+    """``Curl(B1, B2, B3)`` (``src/utils/VectorCalculus.jl:27-50``) on the problem's grid:
+    C2R(im k × R2C(B)) in T; real ``[z][y][x]`` fields in and out."""
+    return tuple(_field_op(prob, _OP["curl"], (B1, B2, B3), 3))
+
+
+def Div(prob, B1, B2, B3):
+    """``Div(B1, B2, B3)`` (``src/utils/VectorCalculus.jl:70-92``): C2R(im k · R2C(B))."""
+    return _field_op(prob, _OP["div"], (B1, B2, B3), 1)[0]
+
+
+def partial_i(prob, B1, direction):
+    """``∂i(B1, direction)`` (``src/utils/VectorCalculus.jl:101-123``), direction "x"/"y"/"z"."""
+    if direction not in ("x", "y", "z"):
+        raise MHDError(-1, "Wrong driection declared")
+    return _field_op(prob, _OP[direction], (B1,), 1)[0]
+
+
+def LaplaceSolver(prob, B):
+    """``LaplaceSolver(B)`` (``src/utils/VectorCalculus.jl:146-168``): C2R(R2C(B) k^-2) with the
+    reference's sign (it returns Φ with -ΔΦ = B for zero-mean B)."""
+    return _field_op(prob, _OP["laplace"], (B,), 1)[0]
+
+
+def VectorPotential(prob, B1, B2, B3):
+    """``VectorPotential(B1, B2, B3)`` (``src/utils/MHDAnalysis.jl:129-180``), Coulomb gauge:
+    C2R(im (k × B̂) k^-2)."""
+    return tuple(_field_op(prob, _OP["vecpot"], (B1, B2, B3), 3))
+
+
+def h_k(prob, iv, jv, kv, L=2 * math.pi):
+    """``h_k(iv, jv, kv; L)`` (``src/utils/MHDAnalysis.jl:94-111``): (curl v · v) dV, the product in
+    T times the Float64 dV = (L/nx)(L/ny)(L/nz) (a Float64 array, as the reference returns)."""
+    g = prob.grid
+    c = Curl(prob, iv, jv, kv)
+    T = prob.T
+    v = [np.asarray(a, dtype=T) for a in (iv, jv, kv)]
+    dV = (L / g.nx) * (L / g.ny) * (L / g.nz)
+    return (c[0] * v[0] + c[1] * v[1] + c[2] * v[2]).astype(T) * dV
+
+
+def h_m(prob, ib, jb, kb):
+    """``h_m(ib, jb, kb)`` (``src/utils/MHDAnalysis.jl:113-117``): A · B pointwise in T (no dV)."""
+    A = VectorPotential(prob, ib, jb, kb)
+    T = prob.T
+    b = [np.asarray(a, dtype=T) for a in (ib, jb, kb)]
+    return A[0] * b[0] + A[1] * b[1] + A[2] * b[2]
+
+
+def getL(prob, iv, jv, kv):
+    """``getL(iv, jv, kv)`` (``src/utils/MHDAnalysis.jl:187-203``): (y kv, z iv, x jv - y iv) with
+    the grid points; pointwise, on the host."""
+    g = prob.grid
+    x, y, z = g.x[None, None, :], g.y[None, :, None], g.z[:, None, None]
+    T = prob.T
+    iv, jv, kv = (np.asarray(a, dtype=T) for a in (iv, jv, kv))
+    return (y * kv - 0), -(0 - z * iv), (x * jv - y * iv)
+
+
 def DivVCorrection(prob):
     """``DivVCorrection!(prob)`` (``src/Solver/VPSolver.jl:100-139``): remove the gradient part
     of u from sol (phi = -i (k . u) / k^2, u -= i k phi) and refresh vars.ux..uz."""
