@@ -290,7 +290,9 @@ class Engine : public EngineBase {
     a.src_f0 = src_f0;
     a.dst_f0 = dst_f0;
     a.ncols = prune ? g.kx_keep : g.nkr;
-    a.nstrips = (a.ncols + S - 1) / S;
+    // strips cover the stored columns: whole lines of the work buffer (see colpass_kernel)
+    const int scols = dst != B_WORK ? a.ncols : (a.ncols >= g.nkr ? g.wrs : g.kx_keep_w);
+    a.nstrips = (std::max(a.ncols, scols) + S - 1) / S;
     if (axis == 2 && prune) {  // only pencils with kept l
       a.nother = nkept_y(g);
       a.other_lo_end = g.ly_lo_end;
