@@ -275,8 +275,9 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None, full_steps=False, 
            "steps": steps, "value": round(steps / wall, 4), "unit": "steps/s", "ms_per_step": round(ms, 2),
            "every_step_full": full_steps, "variants": named,
            "step_roofline": step_roofline(model, steps, ms, exec_call, ms_full, exec_full, dram, src),
-           "kernels": {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4)}
-                       for k in prof},
+           "kernels": {k["name"]: {"launches": k["launches"], "avg_ms": round(k["ms"] / k["launches"], 4),
+                                   "GBps": round(k["bytes"] / k["launches"] / (k["ms"] / k["launches"] * 1e-3) / 1e9, 1)
+                                   if k["bytes"] > 0 else None} for k in prof},
            "finite": not d["has_nan"]}
     del p
     return out
