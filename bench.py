@@ -286,7 +286,9 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None, full_steps=False, 
 def ensemble_bench(M, steps=20):
     """64 runs (16 each of 32^3, 48^3, 64^3, 96^3), Float32, LSRK54, CFL 0.3, seeded
     k^4 exp(-2(k/k0)^2) fields with k0 ~ U[2,6], nu = eta log-uniform in [1e-3, 1e-2]:
-    aggregate run-steps/s of the ensemble vs the same runs stepped one after another."""
+    aggregate run-steps/s of the ensemble (each size group by its own launches on its own
+    stream; then the multi-size modes: all four groups by multi-size launches, and the
+    largest group alone beside the other three) vs the same runs stepped one after another."""
     import numpy as np
     from mhdflows_amd import ensemble as E
     from mhdflows_amd import ic
@@ -302,11 +304,15 @@ def ensemble_bench(M, steps=20):
     for r, (idx, vals) in enumerate(modes):
         ens.set_ic_modes(r, idx, vals)
         ens.set_cfl(r, 0.3)
-    ens.autotune(3)
-    ens.step(2)
-    ens.sync()
-    ms = ens.time_steps(steps)
-    ens_rate = len(specs) * steps / (ms * 1e-3)
+    rates, nmulti, classes = {}, {}, {}
+    for mode in (0, 1, 2):   # group streams (default), multi-size launches, largest group alone
+        nmulti[mode] = ens.multi(mode)
+        ens.autotune(3)
+        classes[mode] = ens.multi_classes()
+        ens.step(2)
+        ens.sync()
+        rates[mode] = len(specs) * steps / (ens.time_steps(steps) * 1e-3)
+    ens_rate = rates[0]
     ens.close()
     # sequential plain path: each run alone (own handle, captured step), one after another
     from mhdflows_amd.devmem import hip
@@ -324,6 +330,13 @@ def ensemble_bench(M, steps=20):
     seq_rate = len(specs) * steps / total
     return {"workload": "64 runs: 16 each of 32^3, 48^3, 64^3, 96^3, Float32 LSRK54, CFL 0.3",
             "steps": steps, "ensemble_run_steps_per_s": round(ens_rate, 1),
+            "mode": "each size group on its own stream (default)",
+            "multi_size": {
+                "all_groups_run_steps_per_s": round(rates[1], 1), "groups": nmulti[1],
+                "largest_group_alone_run_steps_per_s": round(rates[2], 1), "groups_in_multi_launches": nmulti[2],
+                "tuned_classes": {str(m): {"column_passes": "multi" if classes[m][0] == 1 else "per group",
+                                           "row_kernel": "multi" if classes[m][1] == 1 else "per group",
+                                           "spectral": "multi"} for m in (1, 2)}},
             "sequential_run_steps_per_s": round(seq_rate, 1), "speedup": round(ens_rate / seq_rate, 2)}
 
 

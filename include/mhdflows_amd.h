@@ -299,8 +299,12 @@ double mhd_step_bytes_model(const mhd_config* cfg);
 /* ---- batched ensembles: many independent runs stepped together ----------------- */
 typedef struct mhd_batch mhd_batch;
 /* Runs are grouped by (grid size, precision); each group steps with one launch per
- * kernel class covering all its runs (one captured HIP graph per group step), and the
- * groups step concurrently on their own streams. */
+ * kernel class covering all its runs, one captured HIP graph per group step, and the groups
+ * step concurrently on their own streams.  With mhd_batch_multi, groups of the same
+ * precision, physics and stepper whose lengths are all in {32, 48, 64, 96, 128} (and whose
+ * runs have no forcing module or volume penalisation) step together instead: each column
+ * pass, row kernel and spectral update of their step is one multi-size launch covering every
+ * such group, and their whole step is one captured HIP graph. */
 int32_t mhd_batch_create(const mhd_config* cfgs, int32_t nruns, mhd_batch** out);
 int32_t mhd_batch_destroy(mhd_batch* b);
 int32_t mhd_batch_set_ic_real(mhd_batch* b, int32_t run, const void* const* fields);
@@ -314,6 +318,19 @@ int32_t mhd_batch_set_cfl(mhd_batch* b, int32_t run, double cfl_coef);
 /* Advance every run by nsteps steps. */
 int32_t mhd_batch_step(mhd_batch* b, int32_t nsteps);
 int32_t mhd_batch_use_graph(mhd_batch* b, int32_t enable);
+/* Multi-size launches: enable = 1 (the default) steps every eligible group by them, 2 the
+ * same except that the group with the most grid points of each set (of three or more groups)
+ * steps on its own stream beside them, 0 (the default) steps every group on its own stream;
+ * enable < 0 keeps the setting.  ngroups (may be NULL) receives the number of groups stepped by
+ * multi-size launches.  No counterpart in the reference (its ensembles are separate
+ * problems); for ensemble and serving callers. */
+int32_t mhd_batch_multi(mhd_batch* b, int32_t enable, int32_t* ngroups);
+/* Per kernel class of the multi-size sets: column passes (col) and the row kernel (row) as
+ * one multi-size launch (1) or as each group's own launch on the same stream (0); negative
+ * keeps the choice.  mhd_batch_autotune times both forms and keeps the faster; the spectral
+ * update is always one multi-size launch.  out2 (may be NULL) receives the choice of the
+ * first multi-size set, -1 if there is none. */
+int32_t mhd_batch_multi_classes(mhd_batch* b, int32_t col, int32_t row, int32_t* out2);
 int32_t mhd_batch_time_steps(mhd_batch* b, int32_t nsteps, double* ms_out);
 /* Serving-pool control.  A run of a batch is a slot: set its physical parameters,
  * then start it with a fresh clock (t = 0, step = 0), a dt (or CFL coefficient > 0 for
@@ -330,6 +347,8 @@ int32_t mhd_batch_launch(mhd_batch* b, int32_t nsteps);
 int32_t mhd_batch_set_step_pruning(mhd_batch* b, int32_t enable);
 int32_t mhd_batch_query(mhd_batch* b, int32_t* busy);
 int32_t mhd_batch_sync(mhd_batch* b);
+/* Tune every group's launch variants (mhd_autotune), then the multi-size sets' per-class
+ * choice (mhd_batch_multi_classes). */
 int32_t mhd_batch_autotune(mhd_batch* b, int32_t reps);
 
 #ifdef __cplusplus

@@ -1,8 +1,10 @@
 // Engine<T>: launch sequences of the MHD/HD right-hand side and time steps for a
 // group of runs sharing one grid size and precision.
 #include "mhd_engine.hpp"
+#include "mhd_multi.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -280,7 +282,7 @@ class Engine : public EngineBase {
   int colpass(const char* name, int axis, int dir, int src, int dst, int src_f0, int dst_f0, int nfields,
               bool prune, bool hot = false, bool plane_out = false, bool prune_out = false) {
     const int n = axis == 1 ? g.ny : g.nz;
-    const int S = col_strip<T>(n, hot ? col_var[col_class(axis, dir)] : 0);
+    const int S = (rec_ && rec_multi_) ? multi_col_strip(n) : col_strip<T>(n, hot ? col_var[col_class(axis, dir)] : 0);
     ColArgs a{};
     a.g = g;
     a.axis = axis;
@@ -324,6 +326,18 @@ class Engine : public EngineBase {
       bytes = cols * (n - (a.mask_hi - a.mask_lo)) * cxb + cols * n * cxb;
     }
     const cx<T>* tw = axis == 1 ? twy : twz;
+    if (rec_) {
+      Rec r;
+      r.type = Rec::COL;
+      r.n = n;
+      r.dir = dir;
+      r.S = S;
+      r.ca = a;
+      r.runs = cur_rb();
+      r.tw = tw;
+      rec_->push_back(r);
+      return 0;
+    }
     return timed(name, bytes, [&] { return launch_colpass<T>(a, n, dir, S, cur_rb(), tw, st); });
   }
 
@@ -341,7 +355,88 @@ class Engine : public EngineBase {
   }
 
   int xrow(const char* name, const RowArgs& a, int kind, int nfk, double bytes, int var = 0) {
+    if (rec_) {
+      Rec r;
+      r.type = Rec::ROW;
+      r.n = g.nx;
+      r.kind = kind;
+      r.var = var;
+      r.nfk = nfk;
+      r.ra = a;
+      r.runs = cur_rb();
+      rec_->push_back(r);
+      return 0;
+    }
     return timed(name, bytes, [&] { return launch_xrow<T>(a, kind, nfk, var, cur_rb(), twM, twh, st); });
+  }
+
+  // A launch that is not one of the hot kinds: run now, or recorded as a closure over the
+  // stream it is later issued on.
+  template <class F>
+  int other(const char* name, F&& launch) {
+    if (rec_) {
+      Rec r;
+      r.type = Rec::OTHER;
+      r.fn = launch;
+      rec_->push_back(r);
+      return 0;
+    }
+    return timed(name, 0, [&] { return launch(st); });
+  }
+
+  // ---- recording of a time step for multi-size ensemble launches -----------------
+  // While rec_ is set, step_launches_gated() records the hot launches with their arguments
+  // instead of launching them, and the other launches as closures.  With rec_multi_ the
+  // shapes are the multi-size kernels' (column strips of 128 threads, the row plan without
+  // the LDS swizzle); otherwise the group's own tuned shapes.
+  struct Rec {
+    enum { OTHER = 0, COL = 1, ROW = 2, SPEC = 3 };
+    int type = OTHER;
+    int n = 0, dir = 0, kind = 0, var = 0, S = 0, nfk = 0;
+    ColArgs ca{};
+    RowArgs ra{};
+    SpecArgs sa{};
+    const RunBuf<T>* runs = nullptr;
+    const cx<T>* tw = nullptr;
+    std::function<hipError_t(hipStream_t)> fn;
+  };
+  std::vector<Rec>* rec_ = nullptr;
+  bool rec_multi_ = false;
+  uint64_t graph_gen_ = 0;  // bumped whenever the captured step is dropped (its launches changed)
+
+  int record_step(std::vector<Rec>* out, bool multi_shapes) {
+    out->clear();
+    rec_ = out;
+    rec_multi_ = multi_shapes;
+    gate_ = 1;
+    const int rc = step_launches_gated();
+    gate_ = 0;
+    rec_ = nullptr;
+    return rc;
+  }
+  // launch one recorded launch of this group alone (its single-size kernel)
+  hipError_t launch_rec(const Rec& r, hipStream_t s) {
+    switch (r.type) {
+      case Rec::COL: return launch_colpass<T>(r.ca, r.n, r.dir, r.S, r.runs, r.tw, s);
+      case Rec::ROW: return launch_xrow<T>(r.ra, r.kind, r.nfk, r.var, r.runs, twM, twh, s);
+      case Rec::SPEC: return launch_spectral<T>(r.sa, phys, r.runs, s);
+      default: return r.fn(s);
+    }
+  }
+  void reset_prune(hipStream_t s) { (void)launch_small<T>(SK_PRUNE, rbd, nr_, phys, 0, 0, s); }
+  // the group can join multi-size launches: every axis has a multi-size transform and no run
+  // has a calcF! module or volume penalisation (their launches have no multi-size form)
+  bool multi_ok() const override {
+    return multi_length(g.nx) && multi_length(g.ny) && multi_length(g.nz) && !any_module() && !nd_any && !vp_any;
+  }
+  int multi_key() const override { return ((int)sizeof(T) * 4 + phys) * 4 + stepper; }
+  uint64_t graph_gen() const override { return graph_gen_; }
+  // a call's prune marks: the last of its nsteps steps (per run: or its target) is full
+  int mark_prune(int nsteps, hipStream_t s) {
+    if (nsteps <= 0) return 0;
+    const hipError_t e = launch_small<T>(SK_PRUNE, rbd, nr_, phys, nsteps, prune_enabled ? 1 : 0, s);
+    if (e != hipSuccess) return fail(E_HIP, std::string("launch of prune_mark failed: ") + hipGetErrorString(e));
+    return 0;
   }
 
   // ---- autotuning of the hot launches ------------------------------------------
@@ -481,14 +576,7 @@ class Engine : public EngineBase {
       std::lock_guard<std::mutex> lk(g_tab_mu);
       tune_cache()[tune_key()] = best;
     }
-    if (gexec) {
-      hipGraphExecDestroy(gexec);
-      gexec = nullptr;
-    }
-    if (graph) {
-      hipGraphDestroy(graph);
-      graph = nullptr;
-    }
+    drop_graph();
     return 0;
   }
 
@@ -504,14 +592,7 @@ class Engine : public EngineBase {
         if (v5[4] > 3) return fail(E_INVALID, "row variant must be 0..3");
         row_var = v5[4];
       }
-      if (gexec) {
-        hipGraphExecDestroy(gexec);
-        gexec = nullptr;
-      }
-      if (graph) {
-        hipGraphDestroy(graph);
-        graph = nullptr;
-      }
+      drop_graph();
     }
     if (out5) {
       for (int c = 0; c < 4; ++c) out5[c] = col_var[c];
@@ -542,7 +623,9 @@ class Engine : public EngineBase {
     // volume penalisation adds nf products chi/eta (v - V0) (u, and b for MHD) to every one
     const bool nd = nd_any && phys == PHYS_MHD && sp.add_force;
     const bool vp = vp_any;
-    const int rvar = (nd || vp) ? 0 : row_var;
+    // a recorded step keeps the row plan (variant bit 0) and drops the LDS swizzle (bit 1),
+    // which the multi-size row kernel does not have and which changes no value
+    const int rvar = (nd || vp) ? 0 : ((rec_ && rec_multi_) ? (row_var & 1) : row_var);
     const int npf = nd ? 12 : (vp ? np + nf : np);
     RowArgs ra = row_args(rvar);
     ra.in_sel = B_WORK;
@@ -563,7 +646,9 @@ class Engine : public EngineBase {
     MHD_TRY(colpass("zpass_fwd", 2, -1, B_WORK, B_WORK, 0, 0, npf, false, true, false, gate_ != 0));
     if (nd) {
       const double dV = (double)clk_dx(0) * (double)clk_dx(1) * (double)clk_dx(2);
-      MHD_TRY(timed("nd_amp", 0, [&] { return launch_nd_amp<T>(rbd, nr_, ra.blocks_per_run, dV, gate_, st); }));
+      MHD_TRY(other("nd_amp", [this, nb = ra.blocks_per_run, dV, gt = gate_](hipStream_t s) {
+        return launch_nd_amp<T>(rbd, nr_, nb, dV, gt, s);
+      }));
     }
     const int64_t used_blocks = ra.blocks_per_run;
     sp.g = g;
@@ -572,9 +657,17 @@ class Engine : public EngineBase {
     sp.prod_sel = B_WORK;
     sp.gate = gate_;
     sp.planes = fused_planes ? 2 : 0;
-    MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
-    if (last) MHD_TRY(timed("reduce", 0, [&] {
-      return launch_small<T>(SK_REDUCE, rbd, nr_, phys, used_blocks, sp.mode == SP_RHS ? 0 : 1, st);
+    if (rec_) {
+      Rec r;
+      r.type = Rec::SPEC;
+      r.sa = sp;
+      r.runs = rbd;
+      rec_->push_back(r);
+    } else {
+      MHD_TRY(timed("spectral", spectral_bytes(sp), [&] { return launch_spectral<T>(sp, phys, rbd, st); }));
+    }
+    if (last) MHD_TRY(other("reduce", [this, used_blocks, adv = sp.mode == SP_RHS ? 0 : 1](hipStream_t s) {
+      return launch_small<T>(SK_REDUCE, rbd, nr_, phys, used_blocks, adv, s);
     }));
     return 0;
   }
@@ -634,7 +727,7 @@ class Engine : public EngineBase {
   }
 
   int step_launches_gated() {
-    MHD_TRY(timed("cfl", 0, [&] { return launch_small<T>(SK_CFL, rbd, nr_, phys, 0, 0, st); }));
+    MHD_TRY(other("cfl", [this](hipStream_t s) { return launch_small<T>(SK_CFL, rbd, nr_, phys, 0, 0, s); }));
     if (stepper == STEP_RK4) {
       for (int s = 0; s < 4; ++s) {
         SpecArgs sp = spec_base();
@@ -680,10 +773,7 @@ class Engine : public EngineBase {
 
   int step(int nsteps) override {
     if (nsteps < 0) return fail(E_INVALID, "nsteps must be >= 0");
-    if (nsteps > 0) {  // the last of these steps (per run: or its target) is computed in full
-      const hipError_t e = launch_small<T>(SK_PRUNE, rbd, nr_, phys, nsteps, prune_enabled ? 1 : 0, st);
-      if (e != hipSuccess) return fail(E_HIP, std::string("launch of prune_mark failed: ") + hipGetErrorString(e));
-    }
+    MHD_TRY(mark_prune(nsteps, st));  // the last of these steps (per run: or its target) is computed in full
     if (use_graph && !profiling) {
       MHD_TRY(ensure_graph());
       for (int i = 0; i < nsteps; ++i) MHD_HIP(hipGraphLaunch(gexec, st));
@@ -1405,6 +1495,7 @@ class Engine : public EngineBase {
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
+    ++graph_gen_;
   }
   int forcing_done() {
     MHD_TRY(upload_runs());
@@ -1685,6 +1776,304 @@ class Engine : public EngineBase {
     return 0;
   }
 };
+
+// ---------------------------------------------------------------------------
+// Multi-size stepping of several groups of an ensemble (same precision, physics and
+// stepper): each group's step is recorded once, and the recordings are replayed position by
+// position -- one multi-size launch per hot launch (column passes, row kernel, spectral
+// update) for up to MULTI_MAX groups (row launches: groups of one row plan), the small
+// launches (CFL, maxima) per group -- on the first group's stream, captured as one HIP
+// graph per step.
+// ---------------------------------------------------------------------------
+template <class T>
+class FusedStep : public FusedBase {
+ public:
+  using Rec = typename Engine<T>::Rec;
+  std::vector<Engine<T>*> es;
+  // each group's step recorded in the multi-size shapes (recm) and in its own shapes (recs)
+  std::vector<std::vector<Rec>> recm, recs;
+  std::vector<uint64_t> gens;
+  std::vector<hipEvent_t> evs;  // per group: its stream's pending work, then (index 0) our steps
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool use_graph = true;
+  // per kernel class: one multi-size launch for all groups, or each group's own launch
+  // (set by tune(); the spectral update is always multi-size)
+  bool multi_col = true, multi_row = true;
+
+  ~FusedStep() override {
+    if (!es.empty() && es[0]->st) (void)hipStreamSynchronize(es[0]->st);
+    drop();
+    for (hipEvent_t e : evs)
+      if (e) (void)hipEventDestroy(e);
+  }
+  hipStream_t stream() const { return es[0]->st; }
+
+  void drop() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  int init(const std::vector<EngineBase*>& groups) {
+    for (EngineBase* b : groups) es.push_back(static_cast<Engine<T>*>(b));
+    evs.assign(es.size(), nullptr);
+    for (auto& e : evs) MHD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return record();
+  }
+
+  int record() {
+    drop();
+    recm.assign(es.size(), {});
+    recs.assign(es.size(), {});
+    gens.clear();
+    for (size_t k = 0; k < es.size(); ++k) {
+      MHD_TRY(es[k]->record_step(&recm[k], true));
+      MHD_TRY(es[k]->record_step(&recs[k], false));
+      gens.push_back(es[k]->graph_gen());
+    }
+    // every group runs the same launch sequence (same physics and stepper, no modules)
+    for (size_t k = 0; k < es.size(); ++k) {
+      if (recm[k].size() != recm[0].size() || recs[k].size() != recm[0].size())
+        return fail(E_INVALID, "multi-size step: launch sequences differ");
+      for (size_t i = 0; i < recm[0].size(); ++i) {
+        const Rec &a = recm[0][i], &b = recm[k][i], &c = recs[k][i];
+        if (a.type != b.type || a.dir != b.dir || a.kind != b.kind || c.type != a.type)
+          return fail(E_INVALID, "multi-size step: launch sequences differ");
+        if (b.type == Rec::SPEC && (b.sa.planes != 2 || b.sa.modules != 0))
+          return fail(E_INVALID, "multi-size step: spectral launch has no multi-size form");
+      }
+    }
+    return 0;
+  }
+
+  bool stale() const {
+    for (size_t k = 0; k < es.size(); ++k)
+      if (es[k]->graph_gen() != gens[k]) return true;
+    return false;
+  }
+
+  // position i of the step: one multi-size launch per chunk of groups, or every group's own
+  int launch_at(size_t i, hipStream_t s) {
+    const int type = recm[0][i].type;
+    const int ng_all = (int)es.size();
+    const bool multi = type == Rec::SPEC || (type == Rec::COL && multi_col) || (type == Rec::ROW && multi_row);
+    if (!multi) {
+      // the small launches (CFL, maxima) follow the row launches' shapes (block counts)
+      const auto& R = (type == Rec::OTHER && multi_row) ? recm : recs;
+      for (int k = 0; k < ng_all; ++k) {
+        const hipError_t e = es[k]->launch_rec(R[k][i], s);
+        if (e != hipSuccess) return fail(E_HIP, std::string("multi-size step launch: ") + hipGetErrorString(e));
+      }
+      return 0;
+    }
+    // groups that can share a launch (row launches: the same row plan), in chunks of MULTI_MAX
+    for (int pv = 0; pv < 2; ++pv) {
+      std::vector<int> sel;
+      for (int k = 0; k < ng_all; ++k)
+        if (type != Rec::ROW || (recm[k][i].var & 1) == pv) sel.push_back(k);
+      for (size_t c0 = 0; c0 < sel.size(); c0 += MULTI_MAX) {
+        const int ng = (int)std::min<size_t>(MULTI_MAX, sel.size() - c0);
+        hipError_t e = hipSuccess;
+        if (type == Rec::COL) {
+          MultiCol<T> m{};
+          m.ngroups = ng;
+          for (int j = 0; j < ng; ++j) {
+            const Rec& r = recm[sel[c0 + j]][i];
+            m.n[j] = r.n;
+            m.a[j] = r.ca;
+            m.runs[j] = r.runs;
+            m.tw[j] = r.tw;
+            m.start[j + 1] = m.start[j] + r.ca.nitems;
+          }
+          e = launch_colpass_multi<T>(m, recm[sel[c0]][i].dir, s);
+        } else if (type == Rec::ROW) {
+          MultiRow<T> m{};
+          m.ngroups = ng;
+          for (int j = 0; j < ng; ++j) {
+            const int k = sel[c0 + j];
+            const Rec& r = recm[k][i];
+            m.nx[j] = r.n;
+            m.a[j] = r.ra;
+            m.runs[j] = r.runs;
+            m.twM[j] = es[k]->twM;
+            m.twh[j] = es[k]->twh;
+            m.start[j + 1] = m.start[j] + r.ra.blocks_per_run * r.ra.nruns;
+          }
+          e = launch_xrow_multi<T>(m, recm[sel[c0]][i].kind, pv, s);
+        } else {
+          MultiSpec<T> m{};
+          m.ngroups = ng;
+          for (int j = 0; j < ng; ++j) {
+            const Rec& r = recm[sel[c0 + j]][i];
+            const Geom& G = r.sa.g;
+            m.bpx[j] = (int)(((int64_t)G.ny * G.nkr + 255) / 256);
+            m.a[j] = r.sa;
+            m.runs[j] = r.runs;
+            m.start[j + 1] = m.start[j] + (int64_t)m.bpx[j] * G.nz * r.sa.nruns;
+          }
+          e = launch_spectral_multi<T>(m, es[sel[c0]]->phys, s);
+        }
+        if (e != hipSuccess) return fail(E_HIP, std::string("multi-size launch: ") + hipGetErrorString(e));
+      }
+      if (type != Rec::ROW) break;  // one pass over all groups
+    }
+    return 0;
+  }
+
+  int launch_step(hipStream_t s) {
+    for (size_t i = 0; i < recm[0].size(); ++i) MHD_TRY(launch_at(i, s));
+    return 0;
+  }
+
+  int ensure_graph() {
+    if (stale()) MHD_TRY(record());
+    if (gexec) return 0;
+    hipStream_t s = stream();
+    MHD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_step(s);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &gr);
+    if (rc) {
+      if (gr) hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    graph = gr;
+    MHD_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    return 0;
+  }
+
+  // order the fused stream after the other groups' queued work
+  int join_in(hipStream_t s) {
+    for (size_t k = 1; k < es.size(); ++k) {
+      MHD_HIP(hipEventRecord(evs[k], es[k]->st));
+      MHD_HIP(hipStreamWaitEvent(s, evs[k], 0));
+    }
+    return 0;
+  }
+  // later work queued on the other groups' streams waits for the fused stream's work
+  int join_out(hipStream_t s) {
+    MHD_HIP(hipEventRecord(evs[0], s));
+    for (size_t k = 1; k < es.size(); ++k) MHD_HIP(hipStreamWaitEvent(es[k]->st, evs[0], 0));
+    return 0;
+  }
+
+  int step(int nsteps) override {
+    if (nsteps < 0) return fail(E_INVALID, "nsteps must be >= 0");
+    hipStream_t s = stream();
+    MHD_TRY(join_in(s));
+    for (Engine<T>* e : es) MHD_TRY(e->mark_prune(nsteps, s));
+    if (use_graph) {
+      MHD_TRY(ensure_graph());
+      for (int i = 0; i < nsteps; ++i) MHD_HIP(hipGraphLaunch(gexec, s));
+    } else {
+      if (stale()) MHD_TRY(record());
+      for (int i = 0; i < nsteps; ++i) MHD_TRY(launch_step(s));
+    }
+    return join_out(s);
+  }
+
+  int set_graph(int enable) override {
+    use_graph = enable != 0;
+    return 0;
+  }
+
+  // Time the column passes and the row launch of the first right-hand side of a step (they
+  // write only the work buffers; the first stage stores no real fields or maxima) as
+  // multi-size launches and as each group's own launches, with every run's prune marks set
+  // as in the dealiased steps of a call, and keep the faster form per class.
+  int tune(int reps, int* choice) override {
+    if (stale()) MHD_TRY(record());
+    hipStream_t s = stream();
+    MHD_TRY(join_in(s));
+    for (Engine<T>* e : es) {
+      const hipError_t r = launch_small<T>(SK_PRUNE, e->rbd, e->nr_, e->phys, 1 << 20, 1, s);
+      if (r != hipSuccess) return fail(E_HIP, std::string("prune marks: ") + hipGetErrorString(r));
+    }
+    size_t first = 0, end = 0;  // [first, end): the first right-hand side's column and row launches
+    while (first < recm[0].size() && recm[0][first].type == Rec::OTHER) ++first;
+    end = first;
+    while (end < recm[0].size() && recm[0][end].type != Rec::SPEC) ++end;
+    struct Ev {
+      hipEvent_t a = nullptr, b = nullptr;
+      ~Ev() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } ev;
+    MHD_HIP(hipEventCreate(&ev.a));
+    MHD_HIP(hipEventCreate(&ev.b));
+    const bool keep_col = multi_col, keep_row = multi_row;
+    auto time_class = [&](int type, bool multi, float* ms) -> int {
+      (type == Rec::COL ? multi_col : multi_row) = multi;
+      float sum = 0;
+      for (int k = 0; k <= reps; ++k) {
+        MHD_HIP(hipEventRecord(ev.a, s));
+        for (size_t i = first; i < end; ++i)
+          if (recm[0][i].type == type) MHD_TRY(launch_at(i, s));
+        MHD_HIP(hipEventRecord(ev.b, s));
+        MHD_HIP(hipEventSynchronize(ev.b));
+        float e = 0;
+        MHD_HIP(hipEventElapsedTime(&e, ev.a, ev.b));
+        if (k > 0) sum += e;  // the first round warms up
+      }
+      *ms = sum;
+      return 0;
+    };
+    float cm = 0, cs = 0, rm = 0, rs = 0;
+    int rc = time_class(Rec::COL, true, &cm);
+    if (!rc) rc = time_class(Rec::COL, false, &cs);
+    if (!rc) rc = time_class(Rec::ROW, true, &rm);
+    if (!rc) rc = time_class(Rec::ROW, false, &rs);
+    for (Engine<T>* e : es) e->reset_prune(s);
+    if (rc) {
+      multi_col = keep_col;
+      multi_row = keep_row;
+      return rc;
+    }
+    multi_col = cm <= cs;
+    multi_row = rm <= rs;
+    drop();
+    if (choice) {
+      choice[0] = multi_col ? 1 : 0;
+      choice[1] = multi_row ? 1 : 0;
+    }
+    return join_out(s);
+  }
+
+  int set_choice(int col, int row) override {
+    if (col >= 0) multi_col = col != 0;
+    if (row >= 0) multi_row = row != 0;
+    drop();
+    return 0;
+  }
+  void get_choice(int* c2) const override {
+    c2[0] = multi_col ? 1 : 0;
+    c2[1] = multi_row ? 1 : 0;
+  }
+};
+
+FusedBase* make_fused(const std::vector<EngineBase*>& groups, int* err) {
+  *err = 0;
+  if (groups.empty()) return nullptr;
+  FusedBase* f = nullptr;
+  if (groups[0]->precision_bytes() == 4) {
+    auto* x = new FusedStep<float>();
+    *err = x->init(groups);
+    f = x;
+  } else {
+    auto* x = new FusedStep<double>();
+    *err = x->init(groups);
+    f = x;
+  }
+  if (*err) {
+    delete f;
+    return nullptr;
+  }
+  return f;
+}
 
 EngineBase* make_engine(int prec, const Geom& g, const double* L, int phys, int stepper,
                         const std::vector<RunParams>& runs, int device, hipStream_t stream, int* err) {

@@ -100,7 +100,31 @@ class EngineBase {
   virtual int set_variants(const int32_t* v5, int32_t* out5) = 0;
   virtual int sync() = 0;
   virtual hipStream_t stream() const = 0;
+  // multi-size stepping (make_fused): eligibility, the key groups must share (precision,
+  // physics, stepper) and a counter that changes whenever the group's step launches change
+  virtual bool multi_ok() const = 0;
+  virtual int multi_key() const = 0;
+  virtual uint64_t graph_gen() const = 0;
 };
+
+// Several groups of an ensemble stepped together by multi-size launches.
+class FusedBase {
+ public:
+  virtual ~FusedBase() {}
+  // queue nsteps steps of every group (asynchronous; ordered after, and before, the work on
+  // the groups' own streams)
+  virtual int step(int nsteps) = 0;
+  virtual int set_graph(int enable) = 0;
+  // choose per kernel class (column passes, row kernel) between one multi-size launch and
+  // each group's own launch by timing both; choice[0..1] receives 1 for multi-size
+  virtual int tune(int reps, int* choice) = 0;
+  // set the choice directly (negative: keep); get_choice reads it
+  virtual int set_choice(int col, int row) = 0;
+  virtual void get_choice(int* c2) const = 0;
+};
+
+// groups: same multi_key(), all multi_ok(); returns null (with *err set) on failure
+FusedBase* make_fused(const std::vector<EngineBase*>& groups, int* err);
 
 // Device bytes of one run.
 int64_t run_bytes(const Geom& g, int phys, int prec);

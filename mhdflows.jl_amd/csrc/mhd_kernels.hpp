@@ -46,16 +46,17 @@ struct ColArgs {
   int prune_out;       // forward pass of a time step: in a pruned step only kept outputs are produced
 };
 
+// One work item (a strip of S columns of one pencil line) of a column pass, run by the
+// S*(N/E) threads of a workgroup.  lds holds N*S exchange slots and stw N twiddles.  Every
+// early return is taken by the whole workgroup.
 template <class T, int N, int S, int DIR>
-__global__ void __launch_bounds__(S*(N / Plan<N>::E))
-colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void colpass_item(const ColArgs& a, int64_t item, const RunBuf<T>* __restrict__ runs,
+                                             const cx<T>* __restrict__ tw, cx<T>* lds, cx<T>* stw) {
   constexpr int E = Plan<N>::E;
   constexpr int TH = N / E;
   using Rad = typename Plan<N>::R;
   constexpr int R0 = RFirst<Rad>::value;
   constexpr int RL = RLast<Rad>::value;
-  __shared__ cx<T> lds[N * S];
-  __shared__ cx<T> stw[N];  // the twiddle table, staged once per workgroup
 
   const int t = threadIdx.x;
   const int c = t % S;
@@ -70,9 +71,6 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
   const int64_t d_ostride = a.axis == 1 ? dps : drs;
   auto ldsf = [&](int pos) -> cx<T>& { return lds[pos * S + c]; };
 
-  const int64_t nb = (int64_t)gridDim.x;
-  const int64_t item = xcd_item(blockIdx.x, nb);
-  if (item >= a.nitems) return;
   int64_t it = item;
   const int strip = (int)(it % a.nstrips); it /= a.nstrips;
   const int oi = (int)(it % a.nother); it /= a.nother;
@@ -130,6 +128,16 @@ colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __res
       }
     }
   }
+}
+
+template <class T, int N, int S, int DIR>
+__global__ void __launch_bounds__(S*(N / Plan<N>::E))
+colpass_kernel(ColArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ tw) {
+  __shared__ cx<T> lds[N * S];
+  __shared__ cx<T> stw[N];  // the twiddle table, staged once per workgroup
+  const int64_t item = xcd_item(blockIdx.x, (int64_t)gridDim.x);
+  if (item >= a.nitems) return;
+  colpass_item<T, N, S, DIR>(a, item, runs, tw, lds, stw);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,18 +269,35 @@ __device__ __forceinline__ void row_r2c_store(cx<T>* z, int tr, Lds& ldsf, cx<T>
     for (int k = M + 1 + tr; k < kend; k += TX) stst<false, T>(dst + k, cx<T>{T(0), T(0)});
 }
 
-template <class T, int NX, int KIND, int NF, int VAR>
-__global__ void __launch_bounds__(128)
-xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
-            const cx<T>* __restrict__ twh) {
-  constexpr int M = NX / 2;
-  // VAR bit 0 selects the plan (elements per thread), bit 1 the swizzled LDS index
+// Launch shape of the x-row kernel for a row length and variant: VAR bit 0 selects the
+// plan (elements per thread), bit 1 the swizzled LDS index; RPW rows per 128-thread
+// workgroup, each with SLOTS LDS exchange slots.
+template <class T, int NX, int VAR> struct RowShape {
+  static constexpr int M = NX / 2;
   using RP = RowPlanV<T, M, (VAR & 1)>;
-  constexpr bool PAD = (VAR & 2) != 0;
-  constexpr int E = RP::E;
-  constexpr int TX = M / E;
+  static constexpr bool PAD = (VAR & 2) != 0;
+  static constexpr int E = RP::E;
+  static constexpr int TX = M / E;
+  static constexpr int RPW = 128 / TX;
+  static constexpr int SLOTS = row_slots<M, PAD>();
+  static constexpr int LDS_CX = RPW * SLOTS;  // exchange slots of the workgroup
+};
+
+// One work item (RPW rows of one run) of the x-row kernel, run by a 128-thread workgroup.
+// xbuf holds RowShape::LDS_CX exchange slots, red 2*128 values and stw 2*M+1 twiddles.
+// Every early return is taken by the whole workgroup.
+template <class T, int NX, int KIND, int NF, int VAR>
+__device__ __forceinline__ void xrow_item(const RowArgs& a, int64_t item, const RunBuf<T>* __restrict__ runs,
+                                          const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twh,
+                                          cx<T>* xbuf, T* red, cx<T>* stw) {
+  using SH = RowShape<T, NX, VAR>;
+  constexpr int M = SH::M;
+  using RP = typename SH::RP;
+  constexpr bool PAD = SH::PAD;
+  constexpr int E = SH::E;
+  constexpr int TX = SH::TX;
   static_assert(TX <= 128, "row transform too long for one workgroup");
-  constexpr int RPW = 128 / TX;
+  constexpr int RPW = SH::RPW;
   // a row's TX threads are lanes of one wavefront when TX <= 64 (TX is a power of
   // two): its LDS exchanges then need only wavefront-scope ordering
   static_assert((TX & (TX - 1)) == 0, "threads per row must be a power of two");
@@ -290,17 +315,10 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   constexpr int NIN = RowKindInfo<KIND, NF>::NIN;
   constexpr int NPROD = RowKindInfo<KIND, NF>::NPROD;
 
-  __shared__ cx<T> xb[RPW][row_slots<M, PAD>()];
-  __shared__ T red[2][128];
-  __shared__ cx<T> stw[2 * M + 1];  // twiddles: M of the length-M transform, then M+1 of the real pack
-
   const int t = threadIdx.x;
   const int rw = t / TX;
   const int tr = t % TX;
   const Geom& G = a.g;
-  const int64_t nb = (int64_t)gridDim.x;
-  const int64_t item = xcd_item(blockIdx.x, nb);
-  if (item >= a.blocks_per_run * a.nruns) return;
   const int run = (int)(item / a.blocks_per_run);
   const int64_t blk = item % a.blocks_per_run;
   const int64_t row = blk * RPW + rw;
@@ -319,7 +337,7 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
   const cx<T>* const sM = stw;
   const cx<T>* const sh = stw + M;
   const gptr<T> rbase = gmem(a.real_sel ? rb.rscr : rb.vars);
-  cx<T>* xrow = xb[rw];
+  cx<T>* xrow = xbuf + rw * SH::SLOTS;
   auto ldsf = [&](int pos) -> cx<T>& { return xrow[rx<PAD>(pos)]; };
 
   // v[f][e]: the real samples 2j and 2j+1 of field f at the thread's positions
@@ -410,20 +428,21 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
             else mbm = nanmax(nanmax(mbm, s1), s2);
           }
       }
-      red[0][t] = mu;
-      red[1][t] = mbm;
+      T* const red1 = red + 128;
+      red[t] = mu;
+      red1[t] = mbm;
       __syncthreads();
       for (int s = 64; s > 0; s >>= 1) {
         if (t < s) {
-          red[0][t] = nanmax(red[0][t], red[0][t + s]);
-          red[1][t] = nanmax(red[1][t], red[1][t + s]);
+          red[t] = nanmax(red[t], red[t + s]);
+          red1[t] = nanmax(red1[t], red1[t + s]);
         }
         __syncthreads();
       }
       if (t == 0) {
         const gptr<T> part = gmem(rb.partial);
-        part[2 * blk] = red[0][0];
-        part[2 * blk + 1] = red[1][0];
+        part[2 * blk] = red[0];
+        part[2 * blk + 1] = red1[0];
       }
     }
     if constexpr (ND) {
@@ -535,6 +554,19 @@ xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restri
       row_r2c_store<T, M, E, TX, Rfwd, WS, PAD>(z, tr, ldsf, xrow, sM, sh, dst, row_ok, dst_work ? kend_w : kend_c);
     }
   }
+}
+
+template <class T, int NX, int KIND, int NF, int VAR>
+__global__ void __launch_bounds__(128)
+xrow_kernel(RowArgs a, const RunBuf<T>* __restrict__ runs, const cx<T>* __restrict__ twM,
+            const cx<T>* __restrict__ twh) {
+  using SH = RowShape<T, NX, VAR>;
+  __shared__ cx<T> xb[SH::LDS_CX];
+  __shared__ T red[2 * 128];
+  __shared__ cx<T> stw[2 * SH::M + 1];  // twiddles: M of the length-M transform, then M+1 of the real pack
+  const int64_t item = xcd_item(blockIdx.x, (int64_t)gridDim.x);
+  if (item >= a.blocks_per_run * a.nruns) return;
+  xrow_item<T, NX, KIND, NF, VAR>(a, item, runs, twM, twh, xb, red, stw);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,8 +773,8 @@ __device__ __forceinline__ void a99_add(const RunBuf<T>& rb, int kx, const ModeK
 // MODS = 1 adds the calcF! modules (their registers would otherwise cost the plain launch
 // occupancy: 61 -> 72 VGPRs for Float32 MHD)
 template <class T, int PHYS, int PLANES, int MODS>
-__global__ void __launch_bounds__(256)
-spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
+__device__ __forceinline__ void spectral_item(const SpecArgs& a, const RunBuf<T>* __restrict__ runs, uint32_t i,
+                                              int m, int run) {
   constexpr int NF = PHYS == PHYS_MHD ? 6 : 3;
   const Geom& G = a.g;
   // grid: y = m, z = run.  Interior launch (PLANES == 0): x covers the modes
@@ -753,9 +785,6 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
   // Whole launch (PLANES == 2, masked stage input only): x covers every mode of the
   // m slab; the mirror of a kx = 0 mode comes from the plane copy the inverse z-pass
   // made, which this launch does not write, and the masked kx = nx/2 plane is zero.
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int run = blockIdx.z;
-  const int m = blockIdx.y;
   int l, kx;
   if constexpr (PLANES == 2) {
     // dealiased step: a slab of aliased m holds no mode to update
@@ -956,6 +985,12 @@ spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
       }
     }
   }
+}
+
+template <class T, int PHYS, int PLANES, int MODS>
+__global__ void __launch_bounds__(256)
+spectral_kernel(SpecArgs a, const RunBuf<T>* __restrict__ runs) {
+  spectral_item<T, PHYS, PLANES, MODS>(a, runs, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------

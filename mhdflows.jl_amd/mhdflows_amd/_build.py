@@ -15,7 +15,7 @@ CSRC = os.path.normpath(os.path.join(HERE, "..", "csrc"))
 LIB = os.path.join(HERE, "libmhdflows_amd.so")
 OBJDIR = os.path.join(HERE, "..", "_build")
 SOURCES = ["k_col_f32.hip", "k_col_f64.hip", "k_row_f32.hip", "k_row_f64.hip", "k_misc_f32.hip",
-           "k_misc_f64.hip", "mhd_engine.hip", "mhd_api.hip", "mhd_ckpt.hip"]
+           "k_misc_f64.hip", "k_multi_f32.hip", "k_multi_f64.hip", "mhd_engine.hip", "mhd_api.hip", "mhd_ckpt.hip"]
 ARCH = os.environ.get("MHD_OFFLOAD_ARCH", "gfx950")
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -135,6 +135,25 @@ class Ensemble:
     def use_graph(self, enable=True):
         check(lib().mhd_batch_use_graph(self._h, 1 if enable else 0))
 
+    def multi(self, enable=None):
+        """Multi-size launches: True/1 for every eligible group, 2 with the largest group of each
+        set on its own stream, False/0 off, None keeps the setting; returns how many size groups
+        are stepped by them (groups of one precision, physics and stepper with lengths in
+        {32, 48, 64, 96, 128} and no forcing module step as one launch per kernel class)."""
+        n = C.c_int32()
+        mode = -1 if enable is None else int(enable)
+        check(lib().mhd_batch_multi(self._h, mode, C.byref(n)))
+        return n.value
+
+    def multi_classes(self, col=None, row=None):
+        """Column passes / row kernel of the multi-size sets as one multi-size launch (True) or
+        each group's own launch (False); None keeps the choice.  Returns (col, row) of the first
+        set ((-1, -1) if there is none); ``autotune`` sets it by timing both forms."""
+        out = (C.c_int32 * 2)()
+        v = lambda x: -1 if x is None else (1 if x else 0)
+        check(lib().mhd_batch_multi_classes(self._h, v(col), v(row), out))
+        return int(out[0]), int(out[1])
+
     def time_steps(self, nsteps):
         ms = C.c_double()
         check(lib().mhd_batch_time_steps(self._h, int(nsteps), C.byref(ms)))
