@@ -283,7 +283,7 @@ def large_bench(M, n=1024, steps=10, warmup=2, variants=None, full_steps=False, 
     return out
 
 
-def ensemble_bench(M, steps=20):
+def ensemble_bench(M, steps=20, only_mode=None, classes=None):
     """64 runs (16 each of 32^3, 48^3, 64^3, 96^3), Float32, LSRK54, CFL 0.3, seeded
     k^4 exp(-2(k/k0)^2) fields with k0 ~ U[2,6], nu = eta log-uniform in [1e-3, 1e-2]:
     aggregate run-steps/s of the ensemble (each size group by its own launches on its own
@@ -304,16 +304,23 @@ def ensemble_bench(M, steps=20):
     for r, (idx, vals) in enumerate(modes):
         ens.set_ic_modes(r, idx, vals)
         ens.set_cfl(r, 0.3)
-    rates, nmulti, classes = {}, {}, {}
-    for mode in (0, 1, 2):   # group streams (default), multi-size launches, largest group alone
+    rates, nmulti, chosen = {}, {}, {}
+    for mode in ((0, 1, 2) if only_mode is None else (only_mode,)):
+        # group streams (default), multi-size launches, largest group alone
         nmulti[mode] = ens.multi(mode)
         ens.autotune(3)
-        classes[mode] = ens.multi_classes()
+        if classes is not None:   # fixed per-class choice instead of the tuned one
+            ens.multi_classes(*classes)
+        chosen[mode] = ens.multi_classes()
         ens.step(2)
         ens.sync()
         rates[mode] = len(specs) * steps / (ens.time_steps(steps) * 1e-3)
-    ens_rate = rates[0]
     ens.close()
+    if only_mode is not None:   # one mode alone (profiling runs): no sequential baseline
+        return {"workload": "64 runs: 16 each of 32^3, 48^3, 64^3, 96^3, Float32 LSRK54, CFL 0.3",
+                "steps": steps, "multi_mode": only_mode, "groups_in_multi_launches": nmulti[only_mode],
+                "classes": chosen[only_mode], "run_steps_per_s": round(rates[only_mode], 1)}
+    ens_rate = rates[0]
     # sequential plain path: each run alone (own handle, captured step), one after another
     from mhdflows_amd.devmem import hip
     total = 0.0
@@ -334,8 +341,8 @@ def ensemble_bench(M, steps=20):
             "multi_size": {
                 "all_groups_run_steps_per_s": round(rates[1], 1), "groups": nmulti[1],
                 "largest_group_alone_run_steps_per_s": round(rates[2], 1), "groups_in_multi_launches": nmulti[2],
-                "tuned_classes": {str(m): {"column_passes": "multi" if classes[m][0] == 1 else "per group",
-                                           "row_kernel": "multi" if classes[m][1] == 1 else "per group",
+                "tuned_classes": {str(m): {"column_passes": "multi" if chosen[m][0] == 1 else "per group",
+                                           "row_kernel": "multi" if chosen[m][1] == 1 else "per group",
                                            "spectral": "multi"} for m in (1, 2)}},
             "sequential_run_steps_per_s": round(seq_rate, 1), "speedup": round(ens_rate / seq_rate, 2)}
 
@@ -463,6 +470,10 @@ def main():
     ap.add_argument("--large-only", action="store_true",
                     help="run the 1024^3 leg alone (its own JSON line); --steps/--warmup apply to it")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--ensemble-only", type=int, default=None, metavar="MODE",
+                    help="run the 64-run uneven ensemble alone in one multi-size mode (0, 1, 2)")
+    ap.add_argument("--ensemble-classes", type=str, default="",
+                    help="with --ensemble-only: fixed multi-size classes col,row (1 multi, 0 per group)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-steps", type=str, default="100,2000")
@@ -478,6 +489,12 @@ def main():
     import mhdflows_amd as M
     from mhdflows_amd import ic
     from mhdflows_amd.devmem import hip
+
+    if args.ensemble_only is not None:
+        cls = tuple(bool(int(v)) for v in args.ensemble_classes.split(",")) if args.ensemble_classes else None
+        log(f"bench: 64-run uneven ensemble, multi-size mode {args.ensemble_only}")
+        print(json.dumps(ensemble_bench(M, only_mode=args.ensemble_only, classes=cls)), flush=True)
+        return
 
     fixed = [int(v) for v in args.variants.split(",")] if args.variants else None
     if args.large_only:
