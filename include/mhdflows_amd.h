@@ -277,6 +277,12 @@ int32_t mhd_restart(mhd_handle* h, const char* path, int32_t exact);
  * grid and precision (for the process) and rebuild the captured step.  A text
  * report of the timings is written to report (may be NULL). */
 int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_len);
+/* Slab pipeline of each right-hand side: the y-inverse pass, the row kernel and the
+ * y-forward pass run over `slabs` slabs of z-planes as independent chains on two streams, so
+ * one slab's row kernel (arithmetic-bound) overlaps another slab's y passes (memory-bound);
+ * 1 turns it off.  The results are bitwise the same.  No counterpart in the reference (a
+ * launch schedule of this library). */
+int32_t mhd_set_pipeline(mhd_handle* h, int32_t slabs);
 /* Set (v5 != NULL; negative entries keep the current value) and/or read (out5 != NULL)
  * the launch variants {zpass_inv, ypass_inv, ypass_fwd, zpass_fwd: strip width
  * 0 = 128 B, 1 = 64 B, 2 = 256 B, 3 = 512 B of kx per row; xrow_fused: bit 0 plan (Float32

@@ -433,6 +433,10 @@ int32_t mhd_autotune(mhd_handle* h, int32_t reps, char* report, int32_t report_l
   MHD_TRY(check_handle(h));
   return h->e->tune(reps, report, report_len);
 }
+int32_t mhd_set_pipeline(mhd_handle* h, int32_t slabs) {
+  MHD_TRY(check_handle(h));
+  return h->e->set_pipeline(slabs);
+}
 int32_t mhd_set_variants(mhd_handle* h, const int32_t* v5, int32_t* out5) {
   MHD_TRY(check_handle(h));
   return h->e->set_variants(v5, out5);

@@ -102,6 +102,8 @@ class EngineBase {
   virtual hipStream_t stream() const = 0;
   // multi-size stepping (make_fused): eligibility, the key groups must share (precision,
   // physics, stepper) and a counter that changes whenever the group's step launches change
+  // slab pipeline of the right-hand side (1 = off; see Engine::rhs)
+  virtual int set_pipeline(int slabs) = 0;
   virtual bool multi_ok() const = 0;
   virtual int multi_key() const = 0;
   virtual uint64_t graph_gen() const = 0;

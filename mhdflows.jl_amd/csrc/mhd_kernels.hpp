@@ -38,6 +38,7 @@ struct ColArgs {
   int nother;          // other-axis indices processed
   int other_lo_end;    // pruned other set: o < lo_end -> o, else o + other_gap
   int other_gap;
+  int other0;          // first other index (a y pass over a slab of z-planes)
   int mask_lo, mask_hi;  // input positions in [mask_lo, mask_hi) are zero (not read)
   int nruns;
   int64_t nitems;
@@ -76,7 +77,7 @@ __device__ __forceinline__ void colpass_item(const ColArgs& a, int64_t item, con
   const int oi = (int)(it % a.nother); it /= a.nother;
   const int f = (int)(it % a.nfields);
   const int run = (int)(it / a.nfields);
-  const int o = oi < a.other_lo_end ? oi : oi + a.other_gap;
+  const int o = (oi < a.other_lo_end ? oi : oi + a.other_gap) + a.other0;
   const RunBuf<T>& rb = runs[run];
   if (gated_off(a.gate, rb)) return;
   const bool pruned = a.prune_out && step_pruned(rb);
@@ -169,8 +170,9 @@ struct RowArgs {
   int real_out_f0;    // c2r kinds: first real field written
   int write_vars;     // c2r kinds: store the real rows into vars
   int want_max;       // c2r kinds: per-block max of u^2 and b^2 into partial
-  int64_t rows;       // ny*nz
+  int64_t rows;       // end of the rows covered (ny*nz)
   int64_t blocks_per_run;
+  int64_t blk0;       // first row block (a slab of z-planes starts at row blk0 * rows-per-block)
   int gate;           // 1 inside a time step: skip runs whose clock reached its target
   int prune_out;      // products of a time step: a pruned step stores the kept kx only
 };
@@ -320,7 +322,7 @@ __device__ __forceinline__ void xrow_item(const RowArgs& a, int64_t item, const 
   const int tr = t % TX;
   const Geom& G = a.g;
   const int run = (int)(item / a.blocks_per_run);
-  const int64_t blk = item % a.blocks_per_run;
+  const int64_t blk = a.blk0 + item % a.blocks_per_run;  // row block within the run
   const int64_t row = blk * RPW + rw;
   const bool row_ok = row < a.rows;
   const RunBuf<T>& rb = runs[run];

@@ -108,6 +108,7 @@ SIGNATURES = {
     "mhd_batch_step": (_I, [_P, _I]),
     "mhd_batch_use_graph": (_I, [_P, _I]),
     "mhd_batch_multi": (_I, [_P, _I, C.POINTER(_I)]),
+    "mhd_set_pipeline": (_I, [_P, _I]),
     "mhd_batch_multi_classes": (_I, [_P, _I, _I, C.POINTER(_I)]),
     "mhd_batch_time_steps": (_I, [_P, _I, C.POINTER(C.c_double)]),
     "mhd_batch_set_params": (_I, [_P, _I, C.c_double, C.c_double, _I]),

@@ -783,6 +783,12 @@ def variants(prob, set_to=None):
     return list(out)
 
 
+def set_pipeline(prob, slabs):
+    """Slab pipeline of each right-hand side (see ``mhd_set_pipeline``; 1 = off): the same
+    results, the row kernel of one slab overlapping the y passes of another."""
+    check(lib().mhd_set_pipeline(prob._h, int(slabs)))
+
+
 def set_step_pruning(prob, enable=True):
     """Dealiased steps (see ``mhd_set_step_pruning``): every step of a call but the last
     updates kept modes only; the state after the call is the same either way."""
